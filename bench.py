@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark of the Monte Carlo radiation solve (one GPU).
+
+A step is one radiation solve: every emitting cell of every snapshot traced
+with R bundles (ray generation + traversal + tallies + finalisation), inputs
+resident in HBM.  Default workload: ck384x8 = 384x256x256 channel, 8
+temperature snapshots (profile + Fourier modes) in one launch, synthetic
+100x16x69 correlated-k table, 100 rays/cell (2.0e10 rays per step).
+
+Prints ONE JSON line with rays/s (whole job), the traversal kernel's
+algorithmic HBM bandwidth against the MI355X roofline, and the CPU baseline
+(the oracle's restatement of the reference algorithm on this host's cores,
+timed on a bounded sample of emitting cells of the same workload).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "rays/s and wall-clock to 1% RMS Q_rad error vs FP64 oracle (plain vs learned)"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def setup(workload, rays, nsnap=None, mode=0, batch=None):
+    import dnsmc_amd
+    from dnsmc_amd import workloads
+    w = workloads.WORKLOADS[workload]
+    R = rays or w["rays"]
+    ns = nsnap or w.get("nsnap", 1)
+    extra = {"batch_rays": int(batch)} if batch else {}
+    cfg = workloads.channel_config(w["n"], rays_per_cell=R, spectral=w["spectral"], Tw=w["Tw"], nsnap=ns, mode=mode,
+                                   **extra)
+    S = dnsmc_amd.Solver(cfg)
+    t0 = time.time()
+    if w["spectral"] == 3:
+        S.load_tables()
+        table = None
+    else:
+        table = workloads.synth_ck_table(11)
+        S.set_tables(workloads.table_arrays(table))
+    fields = workloads.workload_fields(workload, ns)
+    for s, T in enumerate(fields):
+        S.set_field(s, workloads.ghosted(T))
+    log("setup %s: %.1f s (fields %s x %d)" % (workload, time.time() - t0, w["n"], ns))
+    return S, cfg, fields, table, R
+
+
+def oracle_problem(S, cfg, Tint):
+    n = list(cfg.n)
+    faces = [S.array("faces%d" % a, 0) for a in range(3)]
+    sc = S.array("scalars")
+    d = dict(levels=[dict(n=n, faces=faces, T=np.ascontiguousarray(Tint.transpose(2, 1, 0)).ravel(),
+                          maxs=cfg.maxs[0], centres=[S.array("centres%d" % a, 0) for a in range(3)])],
+             bc=[[cfg.bc[a][s] for s in range(2)] for a in range(3)], polar_axis=cfg.polar_axis,
+             tau_cut=float(sc[10]), spectral=1 if cfg.spectral == 3 else 0, nB=int(sc[3]), nQ=int(sc[4]),
+             nT=int(sc[5]), inv_dT=float(sc[6]), mT0=float(sc[7]), kq=S.array("kq2")[0::2].copy(),
+             Ib=S.array("Ib"), Ibw=S.array("Ibw"), kappa_grey=float(cfg.kappa_grey), cdf_band=S.array("cdf_band"),
+             cdf_g=S.array("cdf_g"), inv_emax=S.array("inv_emax"), Ibmax=float(sc[2]), Ibmax_d=S.Ibmax_d(),
+             Tmax=float(sc[0]), g_search=int(sc[8]), u_off=float(sc[9]))
+    return d
+
+
+def cpu_baseline(S, cfg, fields, R, budget_s):
+    """Oracle (FP32 rays, FP64 tallies, OpenMP over cells) on a seeded sample
+    of emitting cells of snapshot 0; rays/s on this host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    o = pyoracle.get()
+    P = o.problem(oracle_problem(S, cfg, fields[0]))
+    ncell = int(np.prod(cfg.n))
+    rng = np.random.Generator(np.random.Philox(key=7))
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, 64))
+    k = 4 * threads
+    cells = rng.choice(ncell, size=k, replace=False).astype(np.int32)
+    t0 = time.time()
+    o.mc_philox(P, cfg.seed, 0, R, cells=cells, nthreads=threads)
+    dt = time.time() - t0
+    k2 = int(min(ncell, max(k, k * budget_s / max(dt, 1e-3))))
+    cells = rng.choice(ncell, size=k2, replace=False).astype(np.int32)
+    t0 = time.time()
+    _, _, _, nsteps = o.mc_philox(P, cfg.seed, 0, R, cells=cells, nthreads=threads)
+    dt = time.time() - t0
+    rays = k2 * R
+    return dict(value=rays / dt, unit="rays/s", cores=threads, kind="port",
+                sample="%d random emitting cells of snapshot 0 x %d rays (%.1f s, %.3g segments/s)"
+                       % (k2, R, dt, nsteps / dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="ck384x8")
+    ap.add_argument("--rays", type=int, default=None)
+    ap.add_argument("--nsnap", type=int, default=None)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=float, default=None, help="rays per traversal launch")
+    ap.add_argument("--copy-gib", type=float, default=1.0, help="size of the achievable-bandwidth copy (0: skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(emitters shard by construction; see the sharding notes in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import dnsmc_amd
+    S, cfg, fields, table, R = setup(args.workload, args.rays, args.nsnap, batch=args.batch)
+    for _ in range(args.warmup):
+        S.run()
+        S.wait()
+    dnsmc_amd.sync()
+    ms_trace = 0.0
+    ms_gen = 0.0
+    nrays = 0
+    nsteps = 0
+    launches = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        S.run()
+        st = S.stats()   # waits; HIP-event times of this step's kernels
+        ms_trace += st["ms_trace"]
+        ms_gen += st["ms_raygen"]
+        nrays += st["n_rays"]
+        nsteps += st["n_steps"]
+        launches += st["n_trace_launches"]
+    dnsmc_amd.sync()
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    ms_step = 1000.0 * dt / args.steps
+    ncell = int(np.prod(cfg.n)) * cfg.nsnap
+    # algorithmic bytes of the traversal: 4 B per segment (cell T), 4 B per ray
+    # (emitter T), 16 B per cell and snapshot (FP64 E and A written once)
+    b_alg = 4.0 * nsteps + 4.0 * nrays + 16.0 * ncell * args.steps
+    achieved = b_alg / (ms_trace * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": nrays / dt,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "grid": list(cfg.n), "snapshots": cfg.nsnap, "rays_per_cell": R,
+                   "table": "grey kappa=1" if table is None else "synthetic c-k 100x16x69",
+                   "estimator": "plain (Tmax proposal, Philox)", "tally": "fp64"},
+        "segments_per_s": nsteps / dt,
+        "segments_per_ray": nsteps / max(nrays, 1),
+        "ms_trace_per_step": ms_trace / args.steps,
+        "ms_raygen_per_step": ms_gen / args.steps,
+        "trace_launches_per_step": launches // args.steps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "trace", "algorithmic_bytes_per_launch": b_alg / max(launches, 1)},
+        "cpu_baseline": None,
+    }
+    if args.copy_gib > 0:
+        import ctypes
+        from dnsmc_amd._lib import check, lib
+        g = ctypes.c_double(0.0)
+        check(lib().dnsmc_bench_copy(int(args.copy_gib * 2 ** 30), 5, ctypes.byref(g)))
+        out["roofline"]["achievable_copy_GBs"] = g.value
+        out["roofline"]["frac_of_achievable"] = achieved / g.value
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(S, cfg, fields, R, args.cpu_seconds)
+        except Exception as e:  # reported, never fatal to the GPU number
+            log("cpu baseline failed: %r" % (e,))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,12 @@
+"""MI355X-native Monte Carlo radiative-transfer solver for the channel DNS.
+
+The product is libdnsmc_rad.so (HIP kernels for gfx950 + the C ABI of
+include/dnsmc_rad.h).  This package is its host-side mirror: a ctypes
+binding (``Solver``, ``mc_gpu``, ``get_results``), the build script and the
+synthetic workload generators.
+"""
+from .solver import Solver, config_from_dict, default_config, get_results, mc_gpu, parse_config, sync  # noqa: F401
+from ._lib import DnsmcError  # noqa: F401
+
+__all__ = ["Solver", "config_from_dict", "default_config", "get_results", "mc_gpu", "parse_config", "sync",
+           "DnsmcError"]

@@ -1,0 +1,123 @@
+"""ctypes binding of libdnsmc_rad.so (the C ABI in include/dnsmc_rad.h).
+
+The library is the product: there is no Python or CPU fallback.  Loading
+fails loudly when the shared object is missing.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdnsmc_rad.so")
+
+DNSMC_OK, DNSMC_EINVAL, DNSMC_EIO, DNSMC_ERANGE, DNSMC_EDEVICE, DNSMC_ESTATE = 0, -1, -2, -3, -4, -5
+BC_PERIODIC, BC_BLACK = 1, 2
+STRETCH_UNIFORM, STRETCH_REF_QUAD, STRETCH_TANH = 0, 1, 2
+SPEC_GAS_CK, SPEC_GREY_REF, SPEC_PARTICLE, SPEC_GREY_T4 = 0, 1, 2, 3
+MODE_PLAIN, MODE_COMPAT, MODE_LEARNED = 0, 1, 2
+TALLY_F64_ATOMIC, TALLY_F64, TALLY_KAHAN_F32, TALLY_F32 = 0, 1, 2, 3
+
+EXPORTED = [
+    "mc_gpu_", "get_results_", "mc_gpu", "get_results",
+    "dnsmc_default_config", "dnsmc_parse_config", "dnsmc_create", "dnsmc_destroy", "dnsmc_last_error",
+    "dnsmc_load_tables", "dnsmc_set_tables", "dnsmc_set_field", "dnsmc_set_mlp",
+    "dnsmc_set_emitters_slab", "dnsmc_set_emitters_list", "dnsmc_run", "dnsmc_wait",
+    "dnsmc_get_results", "dnsmc_get_results_fortran", "dnsmc_get_stats", "dnsmc_get_array",
+    "dnsmc_get_rays", "dnsmc_get_ray_records", "dnsmc_set_record", "dnsmc_sync", "dnsmc_bench_copy",
+]
+
+
+class DnsmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("dnsmc error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("n", C.c_int * 3),
+        ("L", C.c_float * 3),
+        ("stretch", C.c_int * 3),
+        ("stretch_beta", C.c_float * 3),
+        ("bc", (C.c_int * 2) * 3),
+        ("Tw", (C.c_float * 2) * 3),
+        ("polar_axis", C.c_int),
+        ("nvar", C.c_int),
+        ("photmax", C.c_int),
+        ("rays_per_cell", C.c_int),
+        ("grid_num", C.c_int),
+        ("maxs", C.c_int * 6),
+        ("toll", C.c_float),
+        ("spectral", C.c_int),
+        ("kappa_grey", C.c_float),
+        ("mode", C.c_int),
+        ("sobol", C.c_int),
+        ("num_streams", C.c_int),
+        ("num_gpu", C.c_int), ("gpu", C.c_int), ("p_row", C.c_int),
+        ("seed", C.c_ulonglong),
+        ("nsnap", C.c_int),
+        ("precision", C.c_int),
+        ("tally", C.c_int),
+        ("chunk", C.c_int),
+        ("batch_rays", C.c_longlong),
+        ("table_dir", C.c_char * 512),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("ms_total", C.c_double), ("ms_trace", C.c_double), ("ms_raygen", C.c_double), ("ms_other", C.c_double),
+        ("n_rays", C.c_longlong), ("n_steps", C.c_longlong), ("n_trace_launches", C.c_int),
+        ("Tmax", C.c_float), ("kappamax", C.c_float), ("Ibmax", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load the solver library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libdnsmc_rad.so is not built (run python -m dnsmc_amd.build): %s" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, i32, f32p, f64p = C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_double)
+    L.dnsmc_default_config.argtypes = [C.POINTER(Config)]
+    L.dnsmc_default_config.restype = None
+    L.dnsmc_parse_config.argtypes = [C.c_char_p, C.POINTER(Config)]
+    L.dnsmc_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+    L.dnsmc_destroy.argtypes = [vp]
+    L.dnsmc_destroy.restype = None
+    L.dnsmc_last_error.restype = C.c_char_p
+    L.dnsmc_load_tables.argtypes = [vp]
+    L.dnsmc_set_tables.argtypes = [vp, i32, i32, i32] + [f32p] * 8
+    L.dnsmc_set_field.argtypes = [vp, i32, vp, i32]
+    L.dnsmc_set_mlp.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_float]
+    L.dnsmc_set_emitters_slab.argtypes = [vp, i32]
+    L.dnsmc_set_emitters_list.argtypes = [vp, C.POINTER(C.c_int), i32]
+    L.dnsmc_run.argtypes = [vp]
+    L.dnsmc_wait.argtypes = [vp]
+    L.dnsmc_get_results.argtypes = [vp, i32, f64p, f64p, f64p, f64p]
+    L.dnsmc_get_results_fortran.argtypes = [vp, i32, f32p, f32p]
+    L.dnsmc_get_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.dnsmc_get_array.argtypes = [vp, C.c_char_p, i32, f32p, C.c_longlong]
+    L.dnsmc_get_rays.argtypes = [vp, vp, vp, vp, vp, C.c_longlong]
+    L.dnsmc_get_rays.restype = C.c_longlong
+    L.dnsmc_get_ray_records.argtypes = [vp, vp, vp, vp, vp, vp, C.c_longlong]
+    L.dnsmc_get_ray_records.restype = C.c_longlong
+    L.dnsmc_set_record.argtypes = [vp, i32]
+    L.dnsmc_sync.argtypes = []
+    L.dnsmc_bench_copy.argtypes = [C.c_longlong, i32, f64p]
+    L.mc_gpu_.argtypes = [f32p, C.POINTER(C.c_int)]
+    L.mc_gpu_.restype = None
+    L.get_results_.argtypes = [f32p, f32p]
+    L.get_results_.restype = None
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc < 0:
+        raise DnsmcError(rc, lib().dnsmc_last_error().decode())
+    return rc

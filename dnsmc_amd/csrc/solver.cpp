@@ -1,0 +1,1143 @@
+// Solver orchestration and the C ABI (include/dnsmc_rad.h).
+//
+// Reference counterparts: code/mc_gpu.cu:440-638 (mc_gpu_: grids, T copy,
+// interp3D, Tmax, readT, uploads, launches) and :640-784 (get_results_:
+// synchronise, gather, reorder into the Fortran layout, sdev = sqrt(var)).
+// Differences by design: tables and device state persist across calls (the
+// reference re-reads 121 text files and resets the device every call), the
+// coarse-grid spline runs on the GPU, and errors are returned as codes by the
+// extended API (the two legacy symbols still abort with a message).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "solver.h"
+
+namespace dn {
+// kernels.hip
+template <int SPEC, bool IB_LDS, bool F32TALLY, bool REC, bool AMC>
+__global__ void trace(TraceArgs P);
+__global__ void raygen_plain(GenArgs a);
+__global__ void raygen_compat(GenArgs a);
+__global__ void reduce_chunks(const double* part, const float* partf, int tally, long long nslots, int cps,
+                              long long slot0, double* E, double* A, double* Q2);
+__global__ void finalize(const double* E, const double* A, const double* Q2, long long nemit, int spe, int R,
+                         int compat, double* Q, double* Eo, double* Ao, double* sd);
+__global__ void spline_pass(SplineArgs s, int snap);
+__global__ void fill_f64(double* p, long long n, double v);
+__global__ void copy_f32x4(const float4* src, float4* dst, long long n4);
+}  // namespace dn
+
+using namespace dn;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(DNSMC_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t ensure(size_t cnt)
+    {
+        if (cnt <= n && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(cnt, 1) * sizeof(T));
+        if (e == hipSuccess) n = cnt;
+        return e;
+    }
+    hipError_t upload(const std::vector<T>& v, hipStream_t s)
+    {
+        hipError_t e = ensure(v.size());
+        if (e != hipSuccess) return e;
+        if (v.empty()) return hipSuccess;
+        return hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+};
+
+struct EventPair {
+    hipEvent_t a, b;
+    int kind;   // 0 raygen, 1 trace, 2 other
+};
+
+struct dnsmc_solver {
+    dnsmc_config cfg;
+    Tables tab;
+    bool have_tables = false;
+    std::vector<Level> lev;
+    Derived der;
+    float der_Tmax = -1.f;
+    int dev = 0, ncu = 256;
+    hipStream_t stream = nullptr;
+
+    // fields
+    std::vector<float> Tmax_snap;
+    std::vector<char> field_set;
+    DevBuf<float> d_T[kMaxLev];
+    long long ncell[kMaxLev] = {0, 0, 0, 0, 0, 0};
+    std::vector<float> faces_packed, centres_packed;
+    int foff[kMaxLev][3], coff[3];
+    DevBuf<float> d_faces, d_centres;
+    bool geom_uploaded = false;
+    // spline
+    std::vector<SplineAxis> sp[kMaxLev][3];
+    DevBuf<double> d_spw;
+    std::vector<DevBuf<double>*> sp_dbufs;
+    std::vector<DevBuf<int>*> sp_ibufs;
+    SplineArgs spa[kMaxLev][3];
+    bool spline_ready = false;
+    // tables
+    DevBuf<float> d_kq2, d_Ib, d_Ibw, d_cdfb, d_cdfg, d_inv_emax;
+    DevBuf<double> d_inv_emax_d;
+    // compat
+    DevBuf<uint32_t> d_xw_sub, d_xw_off, d_sobol;
+    bool compat_ready = false;
+    // emitters
+    std::vector<int> emit;
+    std::vector<int> order;     // processing position -> emitter, 3-D brick order
+    DevBuf<int> d_emit, d_order;
+    bool emit_dirty = true;
+    // ray batches are pipelined over kPipes streams, each with its own buffers,
+    // so the tail of one traversal launch overlaps the next batch
+    struct Pipe {
+        hipStream_t stream = nullptr;
+        DevBuf<uint32_t> d_w0, d_w1;
+        DevBuf<float> d_w, d_winv;
+        DevBuf<double> d_part;
+        DevBuf<float> d_partf;
+        hipEvent_t done = nullptr;
+    };
+    static constexpr int kPipes = 3;
+    Pipe pipe[kPipes];
+    hipEvent_t ev_ready = nullptr;
+    // tallies and results
+    DevBuf<double> d_E, d_A, d_Q2, d_Q, d_Eo, d_Ao, d_sd;
+    DevBuf<unsigned long long> d_nsteps;
+    // records
+    int record = 0;
+    long long nrec = 0;
+    DevBuf<int> d_rsteps, d_rend;
+    DevBuf<uint32_t> d_rhash;
+    DevBuf<float> d_rD, d_rG;
+    long long first_batch_rays = 0;
+    // stats
+    std::vector<EventPair> evs;
+    size_t ev_used = 0;
+    dnsmc_stats st;
+    std::chrono::steady_clock::time_point t0;
+    bool running = false;
+    int last_R = 0, last_spe = 1;
+    long long last_nemit = 0;
+    int ystart = 1;
+};
+
+// ------------------------------------------------------------------ util ----
+static int spe_of(const dnsmc_config& c) { return c.mode == DNSMC_MODE_COMPAT ? c.nvar : 1; }
+static int rays_per_slot_of(const dnsmc_config& c) { return c.mode == DNSMC_MODE_COMPAT ? c.photmax : c.rays_per_cell; }
+
+static int check_config(const dnsmc_config& c)
+{
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s)
+            if (c.bc[a][s] != DNSMC_BC_PERIODIC && c.bc[a][s] != DNSMC_BC_BLACK)
+                return fail(DNSMC_EINVAL, "bc must be 1 (periodic) or 2 (black wall)");
+    if (c.polar_axis < 0 || c.polar_axis > 2) return fail(DNSMC_EINVAL, "polar_axis must be 0, 1 or 2");
+    if (c.mode < 0 || c.mode > 2) return fail(DNSMC_EINVAL, "unknown mode");
+    if (c.mode == DNSMC_MODE_LEARNED) return fail(DNSMC_EINVAL, "learned mode needs dnsmc_set_mlp first");
+    if (c.spectral < 0 || c.spectral > 3) return fail(DNSMC_EINVAL, "unknown spectral model");
+    if (c.mode == DNSMC_MODE_COMPAT && (c.nvar < 2 || c.photmax < 1))
+        return fail(DNSMC_EINVAL, "compat mode needs nvar >= 2 and photmax >= 1");
+    if (c.mode != DNSMC_MODE_COMPAT && c.rays_per_cell < 2) return fail(DNSMC_EINVAL, "rays_per_cell must be >= 2");
+    if (!(c.toll > 0.f && c.toll < 1.f)) return fail(DNSMC_EINVAL, "toll must be in (0, 1)");
+    if (c.nsnap < 1) return fail(DNSMC_EINVAL, "nsnap must be >= 1");
+    if (c.tally < 0 || c.tally > 3) return fail(DNSMC_EINVAL, "unknown tally mode");
+    if (c.precision != 0) return fail(DNSMC_EINVAL, "precision 1 (FP64 traversal) is not built in this library");
+    if (c.chunk < 1 || c.chunk > 4096) return fail(DNSMC_EINVAL, "chunk must be in [1, 4096]");
+    if (c.batch_rays < 1024) return fail(DNSMC_EINVAL, "batch_rays must be >= 1024");
+    if (c.spectral == DNSMC_SPEC_GREY_T4 && !(c.kappa_grey > 0.f)) return fail(DNSMC_EINVAL, "kappa_grey must be > 0");
+    if (c.p_row < 1 || c.n[1] % c.p_row) return fail(DNSMC_EINVAL, "p_row must divide n[1]");
+    if (c.mode == DNSMC_MODE_COMPAT) {
+        if (c.num_streams < 1 || c.n[0] % c.num_streams) return fail(DNSMC_EINVAL, "num_streams must divide n[0]");
+        if (c.num_gpu < 1 || c.n[2] % c.num_gpu) return fail(DNSMC_EINVAL, "num_gpu must divide n[2]");
+    }
+    return DNSMC_OK;
+}
+
+static EventPair& next_event(dnsmc_solver* s, int kind)
+{
+    if (s->ev_used == s->evs.size()) {
+        EventPair e;
+        (void)hipEventCreate(&e.a);
+        (void)hipEventCreate(&e.b);
+        s->evs.push_back(e);
+    }
+    EventPair& e = s->evs[s->ev_used++];
+    e.kind = kind;
+    return e;
+}
+
+// ---------------------------------------------------------- geometry ----
+static int upload_geometry(dnsmc_solver* s)
+{
+    if (s->geom_uploaded) return DNSMC_OK;
+    s->faces_packed.clear();
+    for (size_t l = 0; l < s->lev.size(); ++l)
+        for (int a = 0; a < 3; ++a) {
+            s->foff[l][a] = (int)s->faces_packed.size();
+            s->faces_packed.insert(s->faces_packed.end(), s->lev[l].faces[a].begin(), s->lev[l].faces[a].end());
+        }
+    s->centres_packed.clear();
+    for (int a = 0; a < 3; ++a) {
+        s->coff[a] = (int)s->centres_packed.size();
+        s->centres_packed.insert(s->centres_packed.end(), s->lev[0].centres[a].begin(), s->lev[0].centres[a].end());
+    }
+    HIPCHK(s->d_faces.upload(s->faces_packed, s->stream));
+    HIPCHK(s->d_centres.upload(s->centres_packed, s->stream));
+    for (size_t l = 0; l < s->lev.size(); ++l) {
+        s->ncell[l] = (long long)s->lev[l].n[0] * s->lev[l].n[1] * s->lev[l].n[2];
+        HIPCHK(s->d_T[l].ensure((size_t)s->ncell[l] * s->cfg.nsnap));
+    }
+    s->geom_uploaded = true;
+    return DNSMC_OK;
+}
+
+static int prepare_spline(dnsmc_solver* s)
+{
+    if (s->spline_ready || s->lev.size() < 2) return DNSMC_OK;
+    const Level& L0 = s->lev[0];
+    size_t work = 0;
+    for (size_t l = 1; l < s->lev.size(); ++l) {
+        const Level& Lc = s->lev[l];
+        int nf[3] = {L0.n[0], L0.n[1], L0.n[2]}, nc[3] = {Lc.n[0], Lc.n[1], Lc.n[2]};
+        for (int a = 0; a < 3; ++a) {
+            SplineAxis ax;
+            spline_axis_coeffs(L0.centres[a], nf[a], Lc.centres[a], nc[a], ax);
+            SplineArgs& A = s->spa[l][a];
+            memset(&A, 0, sizeof A);
+            auto up_d = [&](const std::vector<double>& v) {
+                DevBuf<double>* b = new DevBuf<double>();
+                (void)b->upload(v, s->stream);
+                s->sp_dbufs.push_back(b);
+                return (const double*)b->p;
+            };
+            DevBuf<int>* kb = new DevBuf<int>();
+            HIPCHK(kb->upload(ax.klo, s->stream));
+            s->sp_ibufs.push_back(kb);
+            A.sig = up_d(ax.sig); A.invp = up_d(ax.invp); A.y2f = up_d(ax.y2f);
+            A.c6 = up_d(ax.c6); A.ihl = up_d(ax.ihl); A.ihh = up_d(ax.ihh);
+            A.ca = up_d(ax.a); A.cb = up_d(ax.b); A.cA = up_d(ax.A); A.cB = up_d(ax.B);
+            A.klo = kb->p;
+            A.nf = nf[a];
+            A.nc = nc[a];
+            if (a == 0) {          // (n0,n1,n2) -> (m0,n1,n2): line l = j + n1 k
+                A.nlines = nf[1] * nf[2];
+                A.A = 1ll << 40; A.sa_in = nf[0]; A.sb_in = 0; A.st_in = 1;
+                A.sa_out = nc[0]; A.sb_out = 0; A.st_out = 1;
+            } else if (a == 1) {   // (m0,n1,n2) -> (m0,m1,n2): line l = i + m0 k
+                A.nlines = nc[0] * nf[2];
+                A.A = nc[0]; A.sa_in = 1; A.sb_in = (long long)nc[0] * nf[1]; A.st_in = nc[0];
+                A.sa_out = 1; A.sb_out = (long long)nc[0] * nc[1]; A.st_out = nc[0];
+            } else {               // (m0,m1,n2) -> (m0,m1,m2): line l = i + m0 j
+                A.nlines = nc[0] * nc[1];
+                A.A = 1ll << 40; A.sa_in = 1; A.sb_in = 0; A.st_in = (long long)nc[0] * nc[1];
+                A.sa_out = 1; A.sb_out = 0; A.st_out = (long long)nc[0] * nc[1];
+            }
+            work = std::max(work, (size_t)A.nlines * A.nf);
+        }
+    }
+    HIPCHK(s->d_spw.ensure(work));
+    s->spline_ready = true;
+    return DNSMC_OK;
+}
+
+// Coarse temperature levels of one snapshot (code/spline.cpp:35-116 on the GPU).
+static int run_spline(dnsmc_solver* s, int snap)
+{
+    if (s->lev.size() < 2) return DNSMC_OK;
+    int rc = prepare_spline(s);
+    if (rc) return rc;
+    const Level& L0 = s->lev[0];
+    static DevBuf<float> tmpx, tmpy;   // per-process scratch for the intermediate passes
+    for (size_t l = 1; l < s->lev.size(); ++l) {
+        const Level& Lc = s->lev[l];
+        size_t nx = (size_t)Lc.n[0] * L0.n[1] * L0.n[2], ny = (size_t)Lc.n[0] * Lc.n[1] * L0.n[2];
+        HIPCHK(tmpx.ensure(nx));
+        HIPCHK(tmpy.ensure(ny));
+        for (int a = 0; a < 3; ++a) {
+            SplineArgs A = s->spa[l][a];
+            A.work = s->d_spw.p;
+            if (a == 0) { A.in = s->d_T[0].p + (size_t)snap * s->ncell[0]; A.out = tmpx.p; }
+            else if (a == 1) { A.in = tmpx.p; A.out = tmpy.p; }
+            else { A.in = tmpy.p; A.out = s->d_T[l].p + (size_t)snap * s->ncell[l]; }
+            A.snap_in = A.snap_out = 0;
+            int nb = (A.nlines + 127) / 128;
+            hipLaunchKernelGGL(spline_pass, dim3(nb), dim3(128), 0, s->stream, A, 0);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    return DNSMC_OK;
+}
+
+// ------------------------------------------------------------ tables ----
+static int ensure_derived(dnsmc_solver* s, float Tmax)
+{
+    if (s->der_Tmax == Tmax) return DNSMC_OK;
+    if (!s->have_tables) return fail(DNSMC_ESTATE, "spectral tables not loaded (dnsmc_load_tables)");
+    std::string err;
+    if (s->tab.synth_grey) synth_grey_tables(s->tab, Tmax);
+    int rc = build_derived(s->cfg, s->tab, Tmax, s->der, &err);
+    if (rc) return fail(rc, err);
+    const Derived& d = s->der;
+    HIPCHK(s->d_kq2.upload(d.kq2, s->stream));
+    HIPCHK(s->d_Ib.upload(d.Ib, s->stream));
+    HIPCHK(s->d_Ibw.upload(d.Ibw, s->stream));
+    HIPCHK(s->d_cdfb.upload(d.cdf_band, s->stream));
+    HIPCHK(s->d_cdfg.upload(d.cdf_g, s->stream));
+    HIPCHK(s->d_inv_emax.upload(d.inv_emax, s->stream));
+    s->der_Tmax = Tmax;
+    return DNSMC_OK;
+}
+
+static int ensure_compat(dnsmc_solver* s)
+{
+    if (s->compat_ready) return DNSMC_OK;
+    std::vector<uint32_t> sub, off;
+    xorwow_jump_tables(sub);
+    // offsets: M^(2^j), j < 64
+    {
+        std::vector<uint32_t> m(160 * 5), t(160 * 5);
+        for (int b = 0; b < 160; ++b) {
+            Xorwow x{0, 0, 0, 0, 0, 0};
+            uint32_t* w = &x.x0;
+            w[b >> 5] = 1u << (b & 31);
+            (void)xorwow_next(x);
+            m[b * 5 + 0] = x.x0; m[b * 5 + 1] = x.x1; m[b * 5 + 2] = x.x2; m[b * 5 + 3] = x.x3; m[b * 5 + 4] = x.x4;
+        }
+        off.assign((size_t)64 * 800, 0u);
+        for (int j = 0; j < 64; ++j) {
+            std::copy(m.begin(), m.end(), off.begin() + (size_t)j * 800);
+            for (int b = 0; b < 160; ++b) {   // square
+                uint32_t r[5] = {0, 0, 0, 0, 0};
+                for (int c = 0; c < 160; ++c)
+                    if ((m[b * 5 + (c >> 5)] >> (c & 31)) & 1u)
+                        for (int w = 0; w < 5; ++w) r[w] ^= m[c * 5 + w];
+                for (int w = 0; w < 5; ++w) t[b * 5 + w] = r[w];
+            }
+            m.swap(t);
+        }
+    }
+    std::vector<uint32_t> sob(64);
+    for (int k = 1; k <= 32; ++k) sob[k - 1] = 1u << (32 - k);   // dimension 0 (van der Corput)
+    uint32_t mk = 1u;                                           // dimension 1: s=1, a=0, m1=1
+    for (int k = 1; k <= 32; ++k) {
+        if (k > 1) mk = (mk << 1) ^ mk;
+        sob[32 + k - 1] = mk << (32 - k);
+    }
+    HIPCHK(s->d_xw_sub.upload(sub, s->stream));
+    HIPCHK(s->d_xw_off.upload(off, s->stream));
+    HIPCHK(s->d_sobol.upload(sob, s->stream));
+    s->compat_ready = true;
+    return DNSMC_OK;
+}
+
+// ---------------------------------------------------------------- run ----
+// Processing order of the emitters: 8x8x8 bricks of cells, bricks and cells
+// inside a brick in (k, j, i) order.  Rays in flight then cover compact 3-D
+// regions (few L2 lines, few TLB pages) instead of thin planes.
+static void brick_order(const std::vector<int>& emit, int n0, int n1, int n2, std::vector<int>& order)
+{
+    const int B = 8;
+    const long long ntot = (long long)n0 * n1 * n2;
+    order.resize(emit.size());
+    bool all = (long long)emit.size() == ntot;
+    for (size_t e = 0; all && e < emit.size(); ++e) all = emit[e] == (int)e;
+    if (all) {   // every cell in natural order: enumerate bricks directly
+        size_t p = 0;
+        for (int bk = 0; bk < n2; bk += B)
+            for (int bj = 0; bj < n1; bj += B)
+                for (int bi = 0; bi < n0; bi += B)
+                    for (int k = bk; k < std::min(bk + B, n2); ++k)
+                        for (int j = bj; j < std::min(bj + B, n1); ++j)
+                            for (int i = bi; i < std::min(bi + B, n0); ++i)
+                                order[p++] = i + n0 * (j + n1 * k);
+        return;
+    }
+    const long long nbx = (n0 + B - 1) / B, nby = (n1 + B - 1) / B;
+    std::vector<std::pair<long long, int>> key(emit.size());
+    for (size_t e = 0; e < emit.size(); ++e) {
+        int c = emit[e];
+        int i = c % n0, j = (c / n0) % n1, k = c / (n0 * n1);
+        long long brick = ((long long)(k / B) * nby + (j / B)) * nbx + (i / B);
+        key[e] = {brick * (B * B * B) + ((k % B) * B + (j % B)) * B + (i % B), (int)e};
+    }
+    std::sort(key.begin(), key.end());
+    for (size_t p = 0; p < emit.size(); ++p) order[p] = key[p].second;
+}
+
+static int largest_divisor_leq(int R, int c)
+{
+    for (int d = std::min(R, c); d >= 1; --d)
+        if (R % d == 0) return d;
+    return 1;
+}
+
+template <int SPEC, bool IB, bool F32, bool REC>
+static void launch_trace_t(bool amc, dim3 g, dim3 b, size_t lds, hipStream_t st, const TraceArgs& P)
+{
+    if (amc) hipLaunchKernelGGL((trace<SPEC, IB, F32, REC, true>), g, b, lds, st, P);
+    else hipLaunchKernelGGL((trace<SPEC, IB, F32, REC, false>), g, b, lds, st, P);
+}
+
+static void launch_trace(int spec, bool ib, bool f32, bool rec, bool amc, dim3 g, dim3 b, size_t lds, hipStream_t st,
+                         const TraceArgs& P)
+{
+    if (spec == kSpecGreyT4) {
+        if (rec) launch_trace_t<kSpecGreyT4, false, false, true>(amc, g, b, lds, st, P);
+        else if (f32) launch_trace_t<kSpecGreyT4, false, true, false>(amc, g, b, lds, st, P);
+        else launch_trace_t<kSpecGreyT4, false, false, false>(amc, g, b, lds, st, P);
+    } else if (ib) {
+        if (rec) launch_trace_t<kSpecTable, true, false, true>(amc, g, b, lds, st, P);
+        else if (f32) launch_trace_t<kSpecTable, true, true, false>(amc, g, b, lds, st, P);
+        else launch_trace_t<kSpecTable, true, false, false>(amc, g, b, lds, st, P);
+    } else {
+        if (rec) launch_trace_t<kSpecTable, false, false, true>(amc, g, b, lds, st, P);
+        else if (f32) launch_trace_t<kSpecTable, false, true, false>(amc, g, b, lds, st, P);
+        else launch_trace_t<kSpecTable, false, false, false>(amc, g, b, lds, st, P);
+    }
+}
+
+static int do_run(dnsmc_solver* s)
+{
+    const dnsmc_config& c = s->cfg;
+    int rc = check_config(c);
+    if (rc) return rc;
+    for (int i = 0; i < c.nsnap; ++i)
+        if (!s->field_set[i]) return fail(DNSMC_ESTATE, "temperature field of snapshot " + std::to_string(i) + " not set");
+    if (s->emit.empty()) return fail(DNSMC_ESTATE, "no emitting cells");
+    float Tmax = 0.f;
+    for (int i = 0; i < c.nsnap; ++i) Tmax = std::max(Tmax, s->Tmax_snap[i]);
+    for (int a = 0; a < 3; ++a)
+        for (int sd = 0; sd < 2; ++sd) Tmax = std::max(Tmax, c.Tw[a][sd]);   // code/mc_gpu.cu:481-486
+    rc = ensure_derived(s, Tmax);
+    if (rc) return rc;
+    if (c.mode == DNSMC_MODE_COMPAT && (rc = ensure_compat(s))) return rc;
+    if (s->emit_dirty) {
+        brick_order(s->emit, c.n[0], c.n[1], c.n[2], s->order);
+        HIPCHK(s->d_emit.upload(s->emit, s->stream));
+        HIPCHK(s->d_order.upload(s->order, s->stream));
+        s->emit_dirty = false;
+    }
+    const Derived& d = s->der;
+    const int spe = spe_of(c), R = rays_per_slot_of(c);
+    const long long nemit = (long long)s->emit.size();
+    const long long nslots = nemit * spe;
+    const int chunk = largest_divisor_leq(R, c.chunk);
+    const long long slots_per_batch = std::max<long long>(1, std::min<long long>(c.batch_rays, 1ll << 30) / R);
+    const long long batch_rays = std::min(slots_per_batch, nslots) * R;
+    const bool det = c.tally != DNSMC_TALLY_F64_ATOMIC;
+    const bool f32 = c.tally == DNSMC_TALLY_KAHAN_F32 || c.tally == DNSMC_TALLY_F32;
+    const long long nchunk_max = (batch_rays + chunk - 1) / chunk;
+    const long long nbatch_total = ((nslots + slots_per_batch - 1) / slots_per_batch) * c.nsnap;
+    const int npipes = (int)std::min<long long>(dnsmc_solver::kPipes, nbatch_total);
+    for (int p = 0; p < npipes; ++p) {
+        dnsmc_solver::Pipe& pp = s->pipe[p];
+        HIPCHK(pp.d_w0.ensure(batch_rays));
+        HIPCHK(pp.d_w1.ensure(batch_rays));
+        HIPCHK(pp.d_w.ensure(batch_rays));
+        HIPCHK(pp.d_winv.ensure(batch_rays));
+        if (det) {
+            if (f32) HIPCHK(pp.d_partf.ensure(3 * nchunk_max));
+            else HIPCHK(pp.d_part.ensure(3 * nchunk_max));
+        }
+    }
+    HIPCHK(s->d_E.ensure(nslots * c.nsnap));
+    HIPCHK(s->d_A.ensure(nslots * c.nsnap));
+    HIPCHK(s->d_Q2.ensure(nslots * c.nsnap));
+    HIPCHK(s->d_Q.ensure(nemit * c.nsnap));
+    HIPCHK(s->d_Eo.ensure(nemit * c.nsnap));
+    HIPCHK(s->d_Ao.ensure(nemit * c.nsnap));
+    HIPCHK(s->d_sd.ensure(nemit * c.nsnap));
+    HIPCHK(s->d_nsteps.ensure(1));
+    if (s->record) {
+        s->nrec = std::min<long long>(batch_rays, 1ll << 22);
+        HIPCHK(s->d_rsteps.ensure(s->nrec));
+        HIPCHK(s->d_rend.ensure(s->nrec));
+        HIPCHK(s->d_rhash.ensure(s->nrec));
+        HIPCHK(s->d_rD.ensure(s->nrec));
+        HIPCHK(s->d_rG.ensure(s->nrec));
+    }
+    s->ev_used = 0;
+    memset(&s->st, 0, sizeof s->st);
+    s->t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipMemsetAsync(s->d_nsteps.p, 0, sizeof(unsigned long long), s->stream));
+    {
+        long long n = nslots * c.nsnap;
+        int nb = (int)((n + 255) / 256);
+        hipLaunchKernelGGL(fill_f64, dim3(nb), dim3(256), 0, s->stream, s->d_E.p, n, 0.0);
+        hipLaunchKernelGGL(fill_f64, dim3(nb), dim3(256), 0, s->stream, s->d_A.p, n, 0.0);
+        hipLaunchKernelGGL(fill_f64, dim3(nb), dim3(256), 0, s->stream, s->d_Q2.p, n, 0.0);
+    }
+    HIPCHK(hipEventRecord(s->ev_ready, s->stream));
+    for (int p = 1; p < npipes; ++p) HIPCHK(hipStreamWaitEvent(s->pipe[p].stream, s->ev_ready, 0));
+    // shared kernel arguments
+    TraceArgs P;
+    memset(&P, 0, sizeof P);
+    P.nlev = (int)s->lev.size();
+    for (int l = 0; l < P.nlev; ++l) {
+        for (int a = 0; a < 3; ++a) {
+            P.lev[l].n[a] = s->lev[l].n[a];
+            P.lev[l].foff[a] = s->foff[l][a];
+        }
+        P.lev[l].maxs = s->lev[l].maxs;
+        P.lev[l].tstride = s->ncell[l];
+        P.T[l] = s->d_T[l].p;
+    }
+    P.bc_periodic = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int sd = 0; sd < 2; ++sd)
+            if (c.bc[a][sd] == DNSMC_BC_PERIODIC) P.bc_periodic |= 1 << (a * 2 + sd);
+    P.polar = c.polar_axis;
+    P.tau_cut = (float)(-log((double)c.toll));
+    P.u_off = d.u_off;
+    P.nB = d.nB; P.nQ = d.nQ; P.nT = d.nT;
+    P.inv_dT = d.inv_dT; P.mT0 = d.mT0;
+    P.kappa_grey = c.kappa_grey;
+    P.kq2 = (const float2*)s->d_kq2.p;
+    P.Ib = s->d_Ib.p;
+    P.Ibw = s->d_Ibw.p;
+    P.faces = s->d_faces.p;
+    P.nfaces = (int)s->faces_packed.size();
+    P.centres = s->d_centres.p;
+    for (int a = 0; a < 3; ++a) P.coff[a] = s->coff[a];
+    P.ncent = (int)s->centres_packed.size();
+    const int spec = c.spectral == DNSMC_SPEC_GREY_T4 ? kSpecGreyT4 : kSpecTable;
+    const bool ib_lds = spec == kSpecTable && (size_t)d.nB * d.nT <= 12288;
+    P.ib_in_lds = ib_lds;
+    P.chunk = chunk;
+    P.rays_per_slot = R;
+    P.slots_per_emitter = spe;
+    P.fd_R = make_fastdiv((uint32_t)R);
+    P.fd_spe = make_fastdiv((uint32_t)spe);
+    P.fd_nx = make_fastdiv((uint32_t)c.n[0]);
+    P.fd_ny = make_fastdiv((uint32_t)c.n[1]);
+    {
+        const char* rk = getenv("DNSMC_REFILL_K");
+        P.refill_k = rk ? atoi(rk) : 16;
+        if (P.refill_k < 1) P.refill_k = 1;
+        if (P.refill_k > 64) P.refill_k = 64;
+    }
+    P.emitters = s->d_emit.p;
+    P.order = s->d_order.p;
+    P.tally = c.tally;
+    P.nsteps = s->d_nsteps.p;
+    P.nrec = s->record ? s->nrec : 0;
+    P.rec_steps = s->d_rsteps.p; P.rec_end = s->d_rend.p; P.rec_hash = s->d_rhash.p;
+    P.rec_D = s->d_rD.p; P.rec_G = s->d_rG.p;
+    P.Tmax = d.Tmax;
+    P.Ibmax_d = d.Ibmax_d;
+    const size_t lds_trace =
+        sizeof(float) * (6 * kLevWords + P.nfaces + P.ncent + 6 * (size_t)d.nB + (ib_lds ? (size_t)d.nB * d.nT : 0));
+    const bool amc = P.nlev > 1;
+
+    GenArgs G;
+    memset(&G, 0, sizeof G);
+    G.T0 = s->d_T[0].p;
+    G.tstride0 = s->ncell[0];
+    G.nB = d.nB; G.nQ = d.nQ; G.nT = d.nT;
+    G.inv_dT = d.inv_dT; G.mT0 = d.mT0;
+    G.kappa_grey = c.kappa_grey;
+    G.spec = spec;
+    G.kq2 = (const float2*)s->d_kq2.p;
+    G.Ib = s->d_Ib.p;
+    G.cdf_band = s->d_cdfb.p;
+    G.cdf_g = s->d_cdfg.p;
+    G.inv_emax = s->d_inv_emax.p;
+    G.Ibmax = d.Ibmax;
+    G.g_search = d.g_search;
+    G.emitters = s->d_emit.p;
+    G.order = s->d_order.p;
+    G.rays_per_slot = R;
+    G.slots_per_emitter = spe;
+    G.seed = c.seed;
+    if (c.mode == DNSMC_MODE_COMPAT) {
+        xorwow_seed_state(c.seed, G.xw_seed);
+        G.xw_jump_sub = s->d_xw_sub.p;
+        G.xw_jump_off = s->d_xw_off.p;
+        G.sobol_v = s->d_sobol.p;
+        G.use_sobol = c.sobol;
+        G.draws_per_bundle = (c.sobol ? 0 : 2) + 1 + (d.g_search ? 1 : 0);
+        G.sub_len = 256;
+        G.n0 = c.n[0]; G.n1 = c.n[1]; G.n2 = c.n[2];
+        G.num_streams = c.num_streams; G.num_gpu = c.num_gpu; G.gpu = c.gpu; G.p_row = c.p_row;
+        G.ystart = s->ystart;
+    }
+    const bool gib = spec == kSpecTable && (size_t)d.nB * d.nT <= 16384;
+    const size_t lds_gen = sizeof(float) * ((size_t)d.nB + (size_t)d.nB * d.nQ + (gib ? (size_t)d.nB * d.nT : 0) + 64);
+
+    const int trace_blocks = s->ncu * 8;
+    s->first_batch_rays = 0;
+    long long bi = 0;
+    for (int snap = 0; snap < c.nsnap; ++snap) {
+        for (long long b0 = 0; b0 < nslots; b0 += slots_per_batch, ++bi) {
+            dnsmc_solver::Pipe& pp = s->pipe[bi % npipes];
+            hipStream_t st = pp.stream;
+            long long nsl = std::min(slots_per_batch, nslots - b0);
+            long long nr = nsl * R;
+            if (snap == 0 && b0 == 0) s->first_batch_rays = nr;
+            G.slot0 = b0;
+            G.nslots = nsl;
+            G.snap = snap;
+            G.w0 = pp.d_w0.p; G.w1 = pp.d_w1.p; G.w = pp.d_w.p; G.winv = pp.d_winv.p;
+            EventPair& eg = next_event(s, 0);
+            HIPCHK(hipEventRecord(eg.a, st));
+            if (c.mode == DNSMC_MODE_COMPAT) {
+                long long nth = nsl * ((R + G.sub_len - 1) / G.sub_len);
+                int nb = (int)std::min<long long>((nth + 255) / 256, 1 << 20);
+                hipLaunchKernelGGL(raygen_compat, dim3(nb), dim3(256), lds_gen, st, G);
+            } else {
+                int nb = (int)std::min<long long>((nr + 255) / 256, (long long)s->ncu * 64);
+                hipLaunchKernelGGL(raygen_plain, dim3(nb), dim3(256), lds_gen, st, G);
+            }
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(eg.b, st));
+            P.w0 = pp.d_w0.p; P.w1 = pp.d_w1.p; P.w = pp.d_w.p; P.winv = pp.d_winv.p;
+            P.part = pp.d_part.p;
+            P.partf = pp.d_partf.p;
+            P.nrays = nr;
+            P.slot0 = b0;
+            P.snap = snap;
+            P.E = s->d_E.p + (size_t)snap * nslots;
+            P.A = s->d_A.p + (size_t)snap * nslots;
+            P.Q2 = s->d_Q2.p + (size_t)snap * nslots;
+            P.nrec = (s->record && snap == 0 && b0 == 0) ? s->nrec : 0;
+            bool rec = P.nrec > 0;
+            long long nch = (nr + chunk - 1) / chunk;
+            int tb = (int)std::min<long long>(trace_blocks, std::max<long long>(1, (nch + 255) / 256));
+            EventPair& et = next_event(s, 1);
+            HIPCHK(hipEventRecord(et.a, st));
+            launch_trace(spec, ib_lds, f32, rec, amc, dim3(tb), dim3(256), lds_trace, st, P);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(et.b, st));
+            s->st.n_trace_launches++;
+            if (det) {
+                int nb = (int)((nsl + 255) / 256);
+                hipLaunchKernelGGL(reduce_chunks, dim3(nb), dim3(256), 0, st, (const double*)pp.d_part.p,
+                                   (const float*)pp.d_partf.p, c.tally, nsl, R / chunk, b0, P.E, P.A, P.Q2);
+                HIPCHK(hipGetLastError());
+            }
+        }
+    }
+    for (int p = 1; p < npipes; ++p) {
+        HIPCHK(hipEventRecord(s->pipe[p].done, s->pipe[p].stream));
+        HIPCHK(hipStreamWaitEvent(s->stream, s->pipe[p].done, 0));
+    }
+    for (int snap = 0; snap < c.nsnap; ++snap) {
+        int nb = (int)((nemit + 255) / 256);
+        hipLaunchKernelGGL(finalize, dim3(nb), dim3(256), 0, s->stream, s->d_E.p + (size_t)snap * nslots,
+                           s->d_A.p + (size_t)snap * nslots, s->d_Q2.p + (size_t)snap * nslots, nemit, spe, R,
+                           c.mode == DNSMC_MODE_COMPAT ? 1 : 0, s->d_Q.p + (size_t)snap * nemit,
+                           s->d_Eo.p + (size_t)snap * nemit, s->d_Ao.p + (size_t)snap * nemit,
+                           s->d_sd.p + (size_t)snap * nemit);
+        HIPCHK(hipGetLastError());
+    }
+    s->st.n_rays = nslots * (long long)R * c.nsnap;
+    s->st.Tmax = d.Tmax;
+    s->st.kappamax = d.kappamax;
+    s->st.Ibmax = d.Ibmax_d;
+    s->last_R = R;
+    s->last_spe = spe;
+    s->last_nemit = nemit;
+    s->running = true;
+    return DNSMC_OK;
+}
+
+static int do_wait(dnsmc_solver* s)
+{
+    if (!s->running) return DNSMC_OK;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->running = false;
+    auto t1 = std::chrono::steady_clock::now();
+    s->st.ms_total = std::chrono::duration<double, std::milli>(t1 - s->t0).count();
+    for (size_t i = 0; i < s->ev_used; ++i) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, s->evs[i].a, s->evs[i].b));
+        if (s->evs[i].kind == 0) s->st.ms_raygen += ms;
+        else if (s->evs[i].kind == 1) s->st.ms_trace += ms;
+        else s->st.ms_other += ms;
+    }
+    unsigned long long ns = 0;
+    HIPCHK(hipMemcpy(&ns, s->d_nsteps.p, sizeof ns, hipMemcpyDeviceToHost));
+    s->st.n_steps = (long long)ns;
+    return DNSMC_OK;
+}
+
+// Rays and records of the first batch are stored in processing order; both
+// getters return them in tally-slot order (emitter-major, then block, then ray).
+static int slot_perm(dnsmc_solver* s, long long c, std::vector<long long>& perm)
+{
+    const int spe = spe_of(s->cfg), R = rays_per_slot_of(s->cfg);
+    perm.resize(c);
+    for (long long q = 0; q < c; ++q) {
+        long long ps = q / R, r = q - ps * R;
+        long long pe = ps / spe, v = ps - pe * spe;
+        perm[q] = ((long long)s->order[pe] * spe + v) * R + r;
+    }
+    return DNSMC_OK;
+}
+
+template <class T>
+static int fetch_perm(const T* dev, long long c, const std::vector<long long>& perm, T* out)
+{
+    if (!out) return DNSMC_OK;
+    std::vector<T> tmp(c);
+    HIPCHK(hipMemcpy(tmp.data(), dev, c * sizeof(T), hipMemcpyDeviceToHost));
+    for (long long q = 0; q < c; ++q)
+        if (perm[q] < c) out[perm[q]] = tmp[q];
+    return DNSMC_OK;
+}
+
+// ================================================================ C ABI ====
+extern "C" {
+
+void dnsmc_default_config(dnsmc_config* cfg) { default_config(cfg); }
+
+int dnsmc_parse_config(const char* text, dnsmc_config* cfg)
+{
+    std::string err;
+    int rc = parse_config(text, cfg, &err);
+    if (rc) return fail(rc, err);
+    return DNSMC_OK;
+}
+
+const char* dnsmc_last_error(void) { return g_err.c_str(); }
+
+int dnsmc_create(const dnsmc_config* cfg, dnsmc_solver** out)
+{
+    if (!cfg || !out) return fail(DNSMC_EINVAL, "null argument");
+    *out = nullptr;
+    int rc = check_config(*cfg);
+    if (rc) return rc;
+    dnsmc_solver* s = new dnsmc_solver();
+    s->cfg = *cfg;
+    std::string err;
+    rc = build_levels(s->cfg, s->lev, &err);
+    if (rc) { delete s; return fail(rc, err); }
+    if (hipGetDevice(&s->dev) != hipSuccess) { delete s; return fail(DNSMC_EDEVICE, "no HIP device"); }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->dev) == hipSuccess && ncu > 0) s->ncu = ncu;
+    if (hipStreamCreate(&s->stream) != hipSuccess) { delete s; return fail(DNSMC_EDEVICE, "hipStreamCreate failed"); }
+    s->pipe[0].stream = s->stream;
+    for (int p = 1; p < dnsmc_solver::kPipes; ++p)
+        if (hipStreamCreate(&s->pipe[p].stream) != hipSuccess) { dnsmc_destroy(s); return fail(DNSMC_EDEVICE, "hipStreamCreate failed"); }
+    for (int p = 0; p < dnsmc_solver::kPipes; ++p)
+        if (hipEventCreateWithFlags(&s->pipe[p].done, hipEventDisableTiming) != hipSuccess) { dnsmc_destroy(s); return fail(DNSMC_EDEVICE, "hipEventCreate failed"); }
+    if (hipEventCreateWithFlags(&s->ev_ready, hipEventDisableTiming) != hipSuccess) { dnsmc_destroy(s); return fail(DNSMC_EDEVICE, "hipEventCreate failed"); }
+    s->Tmax_snap.assign(cfg->nsnap, 0.f);
+    s->field_set.assign(cfg->nsnap, 0);
+    rc = upload_geometry(s);
+    if (rc) { dnsmc_destroy(s); return rc; }
+    // default emitters: every cell
+    long long n = (long long)cfg->n[0] * cfg->n[1] * cfg->n[2];
+    s->emit.resize(n);
+    for (long long i = 0; i < n; ++i) s->emit[i] = (int)i;
+    *out = s;
+    return DNSMC_OK;
+}
+
+void dnsmc_destroy(dnsmc_solver* s)
+{
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto* b : s->sp_dbufs) delete b;
+    for (auto* b : s->sp_ibufs) delete b;
+    for (auto& e : s->evs) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    for (int p = 1; p < dnsmc_solver::kPipes; ++p)
+        if (s->pipe[p].stream) {
+            (void)hipStreamSynchronize(s->pipe[p].stream);
+            (void)hipStreamDestroy(s->pipe[p].stream);
+        }
+    for (int p = 0; p < dnsmc_solver::kPipes; ++p)
+        if (s->pipe[p].done) (void)hipEventDestroy(s->pipe[p].done);
+    if (s->ev_ready) (void)hipEventDestroy(s->ev_ready);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int dnsmc_load_tables(dnsmc_solver* s)
+{
+    if (!s) return fail(DNSMC_EINVAL, "null solver");
+    std::string err;
+    int rc = read_tables(s->cfg, s->tab, &err);
+    if (rc) return fail(rc, err);
+    s->have_tables = true;
+    s->der_Tmax = -1.f;
+    return DNSMC_OK;
+}
+
+int dnsmc_set_tables(dnsmc_solver* s, int nB, int nQ, int nT, const float* Tnb, const float* kP, const float* wvl,
+                     const float* wvr, const float* wvc, const float* kq, const float* prob, const float* probg)
+{
+    if (!s || nB < 1 || nQ < 1 || nT < 2 || !Tnb || !kP || !wvl || !wvr || !wvc || !kq || !prob || !probg)
+        return fail(DNSMC_EINVAL, "dnsmc_set_tables: bad arguments");
+    if (nB > 1024 || nQ > 64) return fail(DNSMC_EINVAL, "at most 1024 bands and 64 quadrature points");
+    Tables& t = s->tab;
+    t = Tables();
+    t.nB = nB; t.nQ = nQ; t.nT = nT;
+    t.Tnb.assign(Tnb, Tnb + nT);
+    t.kP.assign(kP, kP + nT);
+    t.wvl.assign(wvl, wvl + nB);
+    t.wvr.assign(wvr, wvr + nB);
+    t.wvc.assign(wvc, wvc + nB);
+    t.kq.assign(kq, kq + (size_t)nB * nT * nQ);
+    t.prob.assign(prob, prob + (size_t)nT * nB);
+    t.probg.assign(probg, probg + (size_t)nT * nB * nQ);
+    t.kavg.assign(nB, 0.f);
+    for (int b = 0; b < nB; ++b) t.kavg[b] = t.kq[((size_t)b * nT + nT - 1) * nQ];
+    s->have_tables = true;
+    s->der_Tmax = -1.f;
+    return DNSMC_OK;
+}
+
+int dnsmc_set_field(dnsmc_solver* s, int snap, const float* T, int is_device)
+{
+    if (!s || !T) return fail(DNSMC_EINVAL, "null argument");
+    if (snap < 0 || snap >= s->cfg.nsnap) return fail(DNSMC_EINVAL, "snapshot index out of range");
+    if (s->running) {
+        int rc = do_wait(s);
+        if (rc) return rc;
+    }
+    const int n0 = s->cfg.n[0], n1 = s->cfg.n[1], n2 = s->cfg.n[2];
+    const size_t gsz = (size_t)(n0 + 2) * (n1 + 2) * (n2 + 2);
+    std::vector<float> host;
+    const float* src = T;
+    if (is_device) {
+        host.resize(gsz);
+        HIPCHK(hipMemcpy(host.data(), T, gsz * sizeof(float), hipMemcpyDeviceToHost));
+        src = host.data();
+    }
+    // interior (i fastest), Tmax, NaN check (code/mc_gpu.cu:456-480)
+    std::vector<float> in((size_t)n0 * n1 * n2);
+    float tmax = 0.f;
+    size_t o = 0;
+    for (int k = 1; k <= n2; ++k)
+        for (int j = 1; j <= n1; ++j) {
+            const float* row = src + (size_t)(n0 + 2) * (j + (size_t)(n1 + 2) * k);
+            for (int i = 1; i <= n0; ++i) {
+                float v = row[i];
+                if (!(v > 0.f) || !isfinite(v)) return fail(DNSMC_ERANGE, "temperature field holds a non-positive or non-finite value");
+                tmax = v > tmax ? v : tmax;
+                in[o++] = v;
+            }
+        }
+    HIPCHK(hipMemcpyAsync(s->d_T[0].p + (size_t)snap * s->ncell[0], in.data(), in.size() * sizeof(float),
+                          hipMemcpyHostToDevice, s->stream));
+    int rc = run_spline(s, snap);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->Tmax_snap[snap] = tmax;
+    s->field_set[snap] = 1;
+    return DNSMC_OK;
+}
+
+int dnsmc_set_mlp(dnsmc_solver*, const unsigned short*, const float*, const unsigned short*, const float*,
+                  const unsigned short*, const float*, float)
+{
+    return fail(DNSMC_EINVAL, "learned proposal not available in this build");
+}
+
+int dnsmc_set_emitters_slab(dnsmc_solver* s, int ystart)
+{
+    if (!s) return fail(DNSMC_EINVAL, "null solver");
+    const int n0 = s->cfg.n[0], n1 = s->cfg.n[1], n2 = s->cfg.n[2];
+    const int nj = n1 / s->cfg.p_row;
+    if (ystart < 1 || ystart + nj - 1 > n1) return fail(DNSMC_EINVAL, "ystart out of range");
+    s->emit.clear();
+    for (int k = 0; k < n2; ++k)
+        for (int j = ystart - 1; j < ystart - 1 + nj; ++j)
+            for (int i = 0; i < n0; ++i) s->emit.push_back(i + n0 * (j + n1 * k));
+    s->emit_dirty = true;
+    s->ystart = ystart;
+    return DNSMC_OK;
+}
+
+int dnsmc_set_emitters_list(dnsmc_solver* s, const int* cells, int ncells)
+{
+    if (!s || !cells || ncells < 1) return fail(DNSMC_EINVAL, "bad emitter list");
+    long long ntot = (long long)s->cfg.n[0] * s->cfg.n[1] * s->cfg.n[2];
+    for (int i = 0; i < ncells; ++i)
+        if (cells[i] < 0 || cells[i] >= ntot) return fail(DNSMC_EINVAL, "emitter cell out of range");
+    s->emit.assign(cells, cells + ncells);
+    s->emit_dirty = true;
+    s->ystart = 1;
+    return DNSMC_OK;
+}
+
+int dnsmc_run(dnsmc_solver* s)
+{
+    if (!s) return fail(DNSMC_EINVAL, "null solver");
+    if (s->running) {
+        int rc = do_wait(s);
+        if (rc) return rc;
+    }
+    return do_run(s);
+}
+
+int dnsmc_wait(dnsmc_solver* s)
+{
+    if (!s) return fail(DNSMC_EINVAL, "null solver");
+    return do_wait(s);
+}
+
+int dnsmc_get_results(dnsmc_solver* s, int snap, double* Q, double* E, double* A, double* sdev)
+{
+    if (!s) return fail(DNSMC_EINVAL, "null solver");
+    int rc = do_wait(s);
+    if (rc) return rc;
+    if (snap < 0 || snap >= s->cfg.nsnap || s->last_nemit == 0) return fail(DNSMC_ESTATE, "no results for snapshot");
+    size_t n = (size_t)s->last_nemit, o = (size_t)snap * n;
+    if (Q) HIPCHK(hipMemcpy(Q, s->d_Q.p + o, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (E) HIPCHK(hipMemcpy(E, s->d_Eo.p + o, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (A) HIPCHK(hipMemcpy(A, s->d_Ao.p + o, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (sdev) HIPCHK(hipMemcpy(sdev, s->d_sd.p + o, n * sizeof(double), hipMemcpyDeviceToHost));
+    return DNSMC_OK;
+}
+
+// code/mc_gpu.cu:762-776: results into (0:imax+1, 0:jmax/p_row+1, 0:kmax+1).
+int dnsmc_get_results_fortran(dnsmc_solver* s, int snap, float* resfort, float* varfort)
+{
+    if (!s || !resfort || !varfort) return fail(DNSMC_EINVAL, "null argument");
+    size_t n = (size_t)s->last_nemit;
+    std::vector<double> Q(n), sd(n);
+    int rc = dnsmc_get_results(s, snap, Q.data(), nullptr, nullptr, sd.data());
+    if (rc) return rc;
+    const int n0 = s->cfg.n[0], n1 = s->cfg.n[1];
+    const int nj = n1 / s->cfg.p_row;
+    for (size_t e = 0; e < n; ++e) {
+        int cell = s->emit[e];
+        int i = cell % n0 + 1, j = (cell / n0) % n1 + 1 - (s->ystart - 1), k = cell / (n0 * n1) + 1;
+        if (j < 1 || j > nj) continue;
+        size_t f = (size_t)i + (size_t)(n0 + 2) * ((size_t)j + (size_t)(nj + 2) * k);
+        resfort[f] = (float)Q[e];
+        varfort[f] = (float)sd[e];
+    }
+    return DNSMC_OK;
+}
+
+int dnsmc_get_stats(dnsmc_solver* s, dnsmc_stats* st)
+{
+    if (!s || !st) return fail(DNSMC_EINVAL, "null argument");
+    int rc = do_wait(s);
+    if (rc) return rc;
+    *st = s->st;
+    return DNSMC_OK;
+}
+
+int dnsmc_get_array(dnsmc_solver* s, const char* name, int level, float* out, long long n)
+{
+    if (!s || !name) return fail(DNSMC_EINVAL, "null argument");
+    int rc = do_wait(s);
+    if (rc) return rc;
+    std::string nm = name;
+    std::vector<float> v;
+    if (nm == "faces0" || nm == "faces1" || nm == "faces2") {
+        if (level < 0 || level >= (int)s->lev.size()) return fail(DNSMC_EINVAL, "level out of range");
+        v = s->lev[level].faces[nm[5] - '0'];
+    } else if (nm == "centres0" || nm == "centres1" || nm == "centres2") {
+        v = s->lev[level].centres[nm[7] - '0'];
+    } else if (nm == "T") {
+        if (level < 0 || level >= (int)s->lev.size()) return fail(DNSMC_EINVAL, "level out of range");
+        v.resize((size_t)s->ncell[level] * s->cfg.nsnap);
+        HIPCHK(hipMemcpy(v.data(), s->d_T[level].p, v.size() * sizeof(float), hipMemcpyDeviceToHost));
+    } else if (nm == "cdf_band") v = s->der.cdf_band;
+    else if (nm == "cdf_g") v = s->der.cdf_g;
+    else if (nm == "inv_emax") v = s->der.inv_emax;
+    else if (nm == "Ib") v = s->der.Ib;
+    else if (nm == "Ibw") v = s->der.Ibw;
+    else if (nm == "kq2") v = s->der.kq2;
+    else if (nm == "scalars") {
+        const Derived& d = s->der;
+        v = {d.Tmax, d.kappamax, d.Ibmax, (float)d.nB, (float)d.nQ, (float)d.nT, d.inv_dT, d.mT0,
+             (float)d.g_search, d.u_off, (float)(-log((double)s->cfg.toll))};
+    } else if (nm == "Ibmax_d") {
+        double x = s->der.Ibmax_d;
+        if (n >= 2) memcpy(out, &x, sizeof x);
+        return 2;
+    } else return fail(DNSMC_EINVAL, "unknown array " + nm);
+    long long c = std::min<long long>(n, (long long)v.size());
+    if (out && c > 0) memcpy(out, v.data(), c * sizeof(float));
+    return (int)v.size();
+}
+
+long long dnsmc_get_rays(dnsmc_solver* s, unsigned int* w0, unsigned int* w1, float* w, float* winv, long long n)
+{
+    if (!s) return fail(DNSMC_EINVAL, "null solver");
+    int rc = do_wait(s);
+    if (rc) return rc;
+    long long c = s->first_batch_rays;
+    if (c <= 0) return 0;
+    if (s->cfg.nsnap > 1 || (long long)s->emit.size() * spe_of(s->cfg) * rays_per_slot_of(s->cfg) > c)
+        return fail(DNSMC_ESTATE, "ray buffer holds the last batch only; use one batch for ray inspection");
+    if (n < c) return fail(DNSMC_EINVAL, "output arrays must hold every ray of the batch");
+    std::vector<long long> perm;
+    slot_perm(s, c, perm);
+    const dnsmc_solver::Pipe& p = s->pipe[0];
+    if ((rc = fetch_perm<uint32_t>(p.d_w0.p, c, perm, w0))) return rc;
+    if ((rc = fetch_perm<uint32_t>(p.d_w1.p, c, perm, w1))) return rc;
+    if ((rc = fetch_perm<float>(p.d_w.p, c, perm, w))) return rc;
+    if ((rc = fetch_perm<float>(p.d_winv.p, c, perm, winv))) return rc;
+    return c;
+}
+
+long long dnsmc_get_ray_records(dnsmc_solver* s, int* steps, int* end_type, unsigned int* hash, float* D, float* G,
+                                long long n)
+{
+    if (!s) return fail(DNSMC_EINVAL, "null solver");
+    int rc = do_wait(s);
+    if (rc) return rc;
+    if (!s->record) return fail(DNSMC_ESTATE, "records are off (dnsmc_set_record)");
+    long long c = std::min(s->nrec, s->first_batch_rays);
+    if (c <= 0) return 0;
+    if (c < s->first_batch_rays || (long long)s->emit.size() * spe_of(s->cfg) * rays_per_slot_of(s->cfg) > c)
+        return fail(DNSMC_ESTATE, "records cover the whole solve only when it fits one batch of <= 2^22 rays");
+    if (n < c) return fail(DNSMC_EINVAL, "output arrays must hold every record");
+    std::vector<long long> perm;
+    slot_perm(s, c, perm);
+    if ((rc = fetch_perm<int>(s->d_rsteps.p, c, perm, steps))) return rc;
+    if ((rc = fetch_perm<int>(s->d_rend.p, c, perm, end_type))) return rc;
+    if ((rc = fetch_perm<uint32_t>(s->d_rhash.p, c, perm, (uint32_t*)hash))) return rc;
+    if ((rc = fetch_perm<float>(s->d_rD.p, c, perm, D))) return rc;
+    if ((rc = fetch_perm<float>(s->d_rG.p, c, perm, G))) return rc;
+    return c;
+}
+
+int dnsmc_set_record(dnsmc_solver* s, int on)
+{
+    if (!s) return fail(DNSMC_EINVAL, "null solver");
+    s->record = on ? 1 : 0;
+    return DNSMC_OK;
+}
+
+int dnsmc_bench_copy(long long nbytes, int reps, double* gbps)
+{
+    if (nbytes < 16 || reps < 1 || !gbps) return fail(DNSMC_EINVAL, "bad arguments");
+    long long n4 = nbytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc((void**)&a, n4 * 16));
+    HIPCHK(hipMalloc((void**)&b, n4 * 16));
+    HIPCHK(hipMemset(a, 0, n4 * 16));
+    int dev = 0, ncu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(copy_f32x4, dim3(ncu * 8), dim3(256), 0, 0, a, b, n4);   // warm-up
+    HIPCHK(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(copy_f32x4, dim3(ncu * 8), dim3(256), 0, 0, a, b, n4);
+    HIPCHK(hipEventRecord(e1, 0));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *gbps = 2.0 * (double)n4 * 16.0 * reps / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return DNSMC_OK;
+}
+
+int dnsmc_sync(void)
+{
+    HIPCHK(hipDeviceSynchronize());
+    return DNSMC_OK;
+}
+
+// ------------------------------------------------------ legacy boundary ----
+static dnsmc_solver* g_legacy = nullptr;
+static std::mutex g_legacy_mu;
+
+static void legacy_die(const char* where)
+{
+    fprintf(stderr, "Fatal error: %s (%s)\n*** FAILED - ABORTING\n", where, g_err.c_str());
+    exit(1);
+}
+
+static dnsmc_solver* legacy_solver()
+{
+    if (g_legacy) return g_legacy;
+    dnsmc_config c;
+    default_config(&c);
+    const char* path = getenv("DNSMC_CONFIG");
+    if (path && *path) {
+        FILE* fp = fopen(path, "r");
+        if (!fp) {
+            g_err = std::string("cannot open DNSMC_CONFIG file ") + path;
+            legacy_die("mc_gpu_");
+        }
+        std::string text;
+        char buf[4096];
+        size_t k;
+        while ((k = fread(buf, 1, sizeof buf, fp)) > 0) text.append(buf, k);
+        fclose(fp);
+        if (dnsmc_parse_config(text.c_str(), &c)) legacy_die("mc_gpu_: config");
+    }
+    if (dnsmc_create(&c, &g_legacy)) legacy_die("mc_gpu_: create");
+    if (dnsmc_load_tables(g_legacy)) legacy_die("mc_gpu_: tables");
+    return g_legacy;
+}
+
+void mc_gpu_(float* Tfort, int* ystart)
+{
+    std::lock_guard<std::mutex> lk(g_legacy_mu);
+    dnsmc_solver* s = legacy_solver();
+    int ys = ystart ? *ystart : 1;
+    // the DNS passes the full field; emitters are this node's j-slab
+    if (dnsmc_set_field(s, 0, Tfort, 0)) legacy_die("mc_gpu_: field");
+    if (dnsmc_set_emitters_slab(s, ys)) legacy_die("mc_gpu_: ystart");
+    if (dnsmc_run(s)) legacy_die("mc_gpu_: launch");
+}
+
+void get_results_(float* resfort, float* varfort)
+{
+    std::lock_guard<std::mutex> lk(g_legacy_mu);
+    if (!g_legacy) {
+        g_err = "get_results_ called before mc_gpu_";
+        legacy_die("get_results_");
+    }
+    if (dnsmc_get_results_fortran(g_legacy, 0, resfort, varfort)) legacy_die("get_results_");
+}
+
+void mc_gpu(float* Tfort, int* ystart) { mc_gpu_(Tfort, ystart); }
+void get_results(float* resfort, float* varfort) { get_results_(resfort, varfort); }
+
+}  // extern "C"

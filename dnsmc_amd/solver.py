@@ -1,0 +1,190 @@
+"""Host-side mirror of the reference's radiation interface.
+
+``mc_gpu(T, ystart)`` / ``get_results(Q, V)`` keep the argument meaning of
+the reference's Fortran-callable pair (code/mc_gpu.cu:440, :640): T is the
+ghost-inclusive (imax+2, jmax+2, kmax+2) field in Fortran order, the call
+returns before the GPU finishes, and get_results fills the interior of Q and
+of the standard-error array.  ``Solver`` exposes the extended C API
+(runtime configuration, several snapshots per launch, FP64 results).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Config, Stats, check, lib
+
+
+def default_config():
+    c = Config()
+    lib().dnsmc_default_config(C.byref(c))
+    return c
+
+
+def parse_config(text, base=None):
+    c = base if base is not None else default_config()
+    check(lib().dnsmc_parse_config(text.encode(), C.byref(c)))
+    return c
+
+
+def config_from_dict(d, base=None):
+    """Build a Config from a dict of keys of include/dnsmc_rad.h:dnsmc_config."""
+    c = base if base is not None else default_config()
+    for k, v in d.items():
+        if k == "table_dir":
+            c.table_dir = str(v).encode()
+        elif k in ("n", "L", "stretch", "stretch_beta", "maxs"):
+            arr = getattr(c, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        elif k in ("bc", "Tw"):
+            arr = getattr(c, k)
+            for a in range(3):
+                for s in range(2):
+                    arr[a][s] = v[a][s]
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def _f32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Solver:
+    """One solver instance on the current HIP device."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(lib().dnsmc_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.n = tuple(cfg.n)
+        self.nemit = self.n[0] * self.n[1] * self.n[2]
+
+    def close(self):
+        if self.h:
+            lib().dnsmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # tables ---------------------------------------------------------------
+    def load_tables(self):
+        check(lib().dnsmc_load_tables(self.h))
+
+    def set_tables(self, t):
+        """t: dict with Tnb, kP, wvl, wvr, wvc, kq[nB][nT][nQ], prob[nT][nB], probg[nT][nB][nQ]."""
+        arrs = {k: np.ascontiguousarray(t[k], np.float32) for k in
+                ("Tnb", "kP", "wvl", "wvr", "wvc", "kq", "prob", "probg")}
+        nB, nT, nQ = arrs["kq"].shape
+        self._tables = arrs
+        check(lib().dnsmc_set_tables(self.h, nB, nQ, nT, *[_f32(arrs[k]) for k in
+                                                            ("Tnb", "kP", "wvl", "wvr", "wvc", "kq", "prob", "probg")]))
+
+    # field / emitters -----------------------------------------------------
+    def set_field(self, snap, T_ghost):
+        T = np.asfortranarray(T_ghost, dtype=np.float32)
+        if T.shape != (self.n[0] + 2, self.n[1] + 2, self.n[2] + 2):
+            raise ValueError("T must have shape (n0+2, n1+2, n2+2), got %s" % (T.shape,))
+        check(lib().dnsmc_set_field(self.h, snap, T.ctypes.data, 0))
+
+    def set_emitters_slab(self, ystart):
+        check(lib().dnsmc_set_emitters_slab(self.h, ystart))
+        self.nemit = self.n[0] * (self.n[1] // self.cfg.p_row) * self.n[2]
+
+    def set_emitters(self, cells):
+        cells = np.ascontiguousarray(cells, np.int32)
+        check(lib().dnsmc_set_emitters_list(self.h, cells.ctypes.data_as(C.POINTER(C.c_int)), len(cells)))
+        self.nemit = len(cells)
+
+    def set_record(self, on=True):
+        check(lib().dnsmc_set_record(self.h, 1 if on else 0))
+
+    # run ------------------------------------------------------------------
+    def run(self):
+        check(lib().dnsmc_run(self.h))
+
+    def wait(self):
+        check(lib().dnsmc_wait(self.h))
+
+    def results(self, snap=0):
+        out = {k: np.zeros(self.nemit) for k in ("Q", "E", "A", "sdev")}
+        check(lib().dnsmc_get_results(self.h, snap, _f64(out["Q"]), _f64(out["E"]), _f64(out["A"]),
+                                      _f64(out["sdev"])))
+        return out
+
+    def results_fortran(self, snap=0):
+        shp = (self.n[0] + 2, self.n[1] // self.cfg.p_row + 2, self.n[2] + 2)
+        Q = np.zeros(shp, np.float32, order="F")
+        V = np.zeros(shp, np.float32, order="F")
+        check(lib().dnsmc_get_results_fortran(self.h, snap, _f32(Q), _f32(V)))
+        return Q, V
+
+    def stats(self):
+        s = Stats()
+        check(lib().dnsmc_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    # inspection -----------------------------------------------------------
+    def array(self, name, level=0):
+        n = check(lib().dnsmc_get_array(self.h, name.encode(), level, None, 0))
+        out = np.zeros(n, np.float32)
+        check(lib().dnsmc_get_array(self.h, name.encode(), level, _f32(out), n))
+        return out
+
+    def Ibmax_d(self):
+        out = np.zeros(1, np.float64)
+        check(lib().dnsmc_get_array(self.h, b"Ibmax_d", 0, out.ctypes.data_as(C.POINTER(C.c_float)), 2))
+        return float(out[0])
+
+    def rays(self, n):
+        w0 = np.zeros(n, np.uint32)
+        w1 = np.zeros(n, np.uint32)
+        w = np.zeros(n, np.float32)
+        wi = np.zeros(n, np.float32)
+        m = check(lib().dnsmc_get_rays(self.h, w0.ctypes.data, w1.ctypes.data, w.ctypes.data, wi.ctypes.data, n))
+        out = np.zeros(m, dtype=[("w0", "<u4"), ("w1", "<u4"), ("w", "<f4"), ("winv", "<f4")])
+        out["w0"], out["w1"], out["w"], out["winv"] = w0[:m], w1[:m], w[:m], wi[:m]
+        return out
+
+    def records(self, n):
+        st = np.zeros(n, np.int32)
+        en = np.zeros(n, np.int32)
+        hs = np.zeros(n, np.uint32)
+        D = np.zeros(n, np.float32)
+        G = np.zeros(n, np.float32)
+        m = check(lib().dnsmc_get_ray_records(self.h, st.ctypes.data, en.ctypes.data, hs.ctypes.data,
+                                              D.ctypes.data, G.ctypes.data, n))
+        return {"steps": st[:m], "end": en[:m], "hash": hs[:m], "D": D[:m], "G": G[:m]}
+
+
+# ----------------------------------------------------- reference mirror ----
+def mc_gpu(T, ystart=1):
+    """code/mc_gpu.cu:440 mc_gpu_(T, ystart): asynchronous launch (copies T)."""
+    T = np.asfortranarray(T, dtype=np.float32)
+    ys = C.c_int(ystart)
+    lib().mc_gpu_(_f32(T), C.byref(ys))
+
+
+def get_results(Q, V):
+    """code/mc_gpu.cu:640 get_results_(Q, V): wait and fill the interior."""
+    if not (Q.flags.f_contiguous and V.flags.f_contiguous and Q.dtype == np.float32 and V.dtype == np.float32):
+        raise ValueError("Q and V must be Fortran-ordered float32 arrays")
+    lib().get_results_(_f32(Q), _f32(V))
+
+
+def sync():
+    check(lib().dnsmc_sync())
+
+
+__all__ = ["Solver", "default_config", "parse_config", "config_from_dict", "mc_gpu", "get_results", "sync",
+           "_lib"]

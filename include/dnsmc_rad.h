@@ -1,0 +1,187 @@
+/*
+ * dnsmc_rad.h -- C ABI of the MI355X Monte Carlo radiative-transfer solver.
+ *
+ * The channel DNS (Fortran) computes the radiative source term Q_rad [W/m^3]
+ * from its temperature field through two calls.  This library exports exactly
+ * those symbols, with the same argument meaning, array layout and
+ * asynchronous contract, so it replaces the reference's mc_gpu.o, memory_copy.o,
+ * read_tables.o and spline.o plus -lcudart -lcurand on the link line
+ * (code/make_test:22-23, code/makefile:21-22).
+ *
+ * Everything in the signatures is a plain pointer or size; nothing depends on
+ * torch or on any C++ type.  All extended entry points return 0 on success and
+ * a negative DNSMC_E* code on failure; dnsmc_last_error() gives the message.
+ */
+#ifndef DNSMC_RAD_H
+#define DNSMC_RAD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Reference boundary (Fortran implicit interface: lower case + trailing _).
+ * ------------------------------------------------------------------------ */
+
+/* Replaces code/mc_gpu.cu:440  extern "C" void mc_gpu_(float *Tfort, int *ystart).
+ * Tfort: real(4) T(0:imax+1, 0:jmax+1, 0:kmax+1), column-major (i fastest),
+ *        ghost layers included, in K.  Copied before return: the caller may
+ *        overwrite it immediately.
+ * ystart: 1-based first j of this node's slab (xstart(2) in 2DECOMP).
+ * Returns as soon as the work is queued on the GPU (the DNS overlaps several
+ * time steps with it, code/main_comp.f:151-208).  Sizes, domain and physics
+ * come from the runtime configuration (DNSMC_CONFIG file; defaults equal the
+ * reference's shipped code/param.h).  Errors abort with a message, as the
+ * reference's cudaCheckErrors does (code/param.h:85-95). */
+void mc_gpu_(float *Tfort, int *ystart);
+
+/* Replaces code/mc_gpu.cu:640  extern "C" void get_results_(float *resfort, float *varfort).
+ * Waits for the work queued by mc_gpu_, then writes interior cells of
+ * resfort = Q_rad [W/m^3] (positive = net emission) and varfort = MC standard
+ * error of Q_rad, both real(4) (0:imax+1, 0:jmax/p_row+1, 0:kmax+1),
+ * column-major; ghost entries are left untouched. */
+void get_results_(float *resfort, float *varfort);
+
+/* BIND(C, NAME='mc_gpu') / NAME='get_results' spellings of the same calls:
+ * the reference declares (but does not use) this iso_c_binding interface in
+ * code/rad_main.f:2-9. */
+void mc_gpu(float *Tfort, int *ystart);
+void get_results(float *resfort, float *varfort);
+
+/* ------------------------------------------------------------------------
+ * Extended API.  No counterpart in the reference (its configuration is
+ * compile-time macros in code/param.h and code/NarrowBand.h); the two calls
+ * above are thin wrappers over it.
+ * ------------------------------------------------------------------------ */
+
+enum {
+    DNSMC_OK = 0,
+    DNSMC_EINVAL = -1,      /* bad argument / configuration */
+    DNSMC_EIO = -2,         /* table file missing or malformed */
+    DNSMC_ERANGE = -3,      /* temperature outside the spectral table, NaN in T */
+    DNSMC_EDEVICE = -4,     /* HIP runtime error */
+    DNSMC_ESTATE = -5       /* calls out of order */
+};
+
+enum { DNSMC_BC_PERIODIC = 1, DNSMC_BC_BLACK = 2 };
+enum { DNSMC_STRETCH_UNIFORM = 0, DNSMC_STRETCH_REF_QUAD = 1, DNSMC_STRETCH_TANH = 2 };
+/* spectral models: code/NarrowBand.h grey==0/1/2, plus a constant-kappa grey gas */
+enum { DNSMC_SPEC_GAS_CK = 0, DNSMC_SPEC_GREY_REF = 1, DNSMC_SPEC_PARTICLE = 2, DNSMC_SPEC_GREY_T4 = 3 };
+/* estimators */
+enum {
+    DNSMC_MODE_PLAIN = 0,    /* Philox rays, proposal built exactly at Tmax */
+    DNSMC_MODE_COMPAT = 1,   /* reference sampling: XORWOW per (cell, block), Sobol directions,
+                                proposal CDF at the table node nearest Tmax */
+    DNSMC_MODE_LEARNED = 2   /* MLP proposal over direction bins and (band, g), p/q reweighted */
+};
+enum { DNSMC_TALLY_F64_ATOMIC = 0, DNSMC_TALLY_F64 = 1, DNSMC_TALLY_KAHAN_F32 = 2, DNSMC_TALLY_F32 = 3 };
+
+typedef struct dnsmc_config {
+    int n[3];                /* interior cells imax, jmax, kmax */
+    float L[3];              /* Lx, Ly, Lz [m] */
+    int stretch[3];          /* face generator per axis (DNSMC_STRETCH_*) */
+    float stretch_beta[3];   /* tanh factor for DNSMC_STRETCH_TANH */
+    int bc[3][2];            /* per axis, low/high side: DNSMC_BC_* */
+    float Tw[3][2];          /* wall temperatures [K] */
+    int polar_axis;          /* axis of cos(theta) for emitted directions */
+    int nvar;                /* variance blocks (compat) */
+    int photmax;             /* bundles per block and cell (compat) */
+    int rays_per_cell;       /* plain / learned estimators */
+    int grid_num;            /* adaptive mesh coarsening levels (1 = off) */
+    int maxs[6];             /* step cap per level */
+    float toll;              /* transmissivity cut-off */
+    int spectral;            /* DNSMC_SPEC_* */
+    float kappa_grey;        /* DNSMC_SPEC_GREY_T4 absorption coefficient [1/m] */
+    int mode;                /* DNSMC_MODE_* */
+    int sobol;               /* compat: Sobol directions (1) or XORWOW (0) */
+    int num_streams;         /* compat seeding: reference stream count (code/param.h:40) */
+    int num_gpu, gpu, p_row; /* compat seeding and slab decomposition */
+    unsigned long long seed;
+    int nsnap;               /* temperature snapshots batched in one run */
+    int precision;           /* 0 FP32 traversal, 1 FP64 traversal */
+    int tally;               /* DNSMC_TALLY_* */
+    int chunk;               /* rays per lane chunk in the traversal */
+    long long batch_rays;    /* rays generated per traversal launch */
+    char table_dir[512];     /* directory of the spectral tables */
+} dnsmc_config;
+
+typedef struct dnsmc_solver dnsmc_solver;
+
+typedef struct dnsmc_stats {
+    double ms_total;          /* last run, host wall clock */
+    double ms_trace;          /* sum of traversal kernel time (HIP events) */
+    double ms_raygen;         /* sum of ray generation kernel time */
+    double ms_other;          /* spline, finalize, reductions */
+    long long n_rays;
+    long long n_steps;        /* cell crossings (segments) */
+    int n_trace_launches;
+    float Tmax;
+    float kappamax;
+    double Ibmax;
+} dnsmc_stats;
+
+/* Defaults equal to the reference's shipped code/param.h + NarrowBand.h. */
+void dnsmc_default_config(dnsmc_config *cfg);
+/* Parse "key = value" lines (see dnsmc_amd/config.py for the keys). */
+int dnsmc_parse_config(const char *text, dnsmc_config *cfg);
+
+int dnsmc_create(const dnsmc_config *cfg, dnsmc_solver **out);
+void dnsmc_destroy(dnsmc_solver *s);
+const char *dnsmc_last_error(void);
+
+/* Spectral tables.  dnsmc_load_tables reads the reference text formats from
+ * cfg->table_dir (code/read_tables.cpp:20-92 gas, :187-241 particle;
+ * grey==1 and grey-T4 tables are synthesised).  dnsmc_set_tables takes the
+ * same data as arrays: Tnb[nT], kP[nT], wvl/wvr/wvc[nB], kq[nB][nT][nQ],
+ * prob[nT][nB], probg[nT][nB][nQ], gq/wq[nQ] (gq, wq may be NULL). */
+int dnsmc_load_tables(dnsmc_solver *s);
+int dnsmc_set_tables(dnsmc_solver *s, int nB, int nQ, int nT, const float *Tnb, const float *kP,
+                     const float *wvl, const float *wvr, const float *wvc, const float *kq,
+                     const float *prob, const float *probg);
+
+/* Temperature field of snapshot `snap`, Fortran layout with ghosts
+ * (n0+2)(n1+2)(n2+2), host or device pointer (is_device). */
+int dnsmc_set_field(dnsmc_solver *s, int snap, const float *T, int is_device);
+/* Learned proposal weights (BF16 bits, row-major [in][out]) + FP32 biases. */
+int dnsmc_set_mlp(dnsmc_solver *s, const unsigned short *W1, const float *b1, const unsigned short *W2,
+                  const float *b2, const unsigned short *W3, const float *b3, float mix_eps);
+
+/* Emitting cells: the j-slab [ystart, ystart + n1/p_row) (1-based) as in the
+ * reference, or an explicit list of 0-based linear cells (i fastest). */
+int dnsmc_set_emitters_slab(dnsmc_solver *s, int ystart);
+int dnsmc_set_emitters_list(dnsmc_solver *s, const int *cells, int ncells);
+
+/* Queue the whole radiation solve (all snapshots) and return. */
+int dnsmc_run(dnsmc_solver *s);
+/* Wait for the queued solve. */
+int dnsmc_wait(dnsmc_solver *s);
+/* Results of snapshot `snap` for the emitters, in emitter order (FP64):
+ * Q = E - A, E, A and the MC standard error; any pointer may be NULL. */
+int dnsmc_get_results(dnsmc_solver *s, int snap, double *Q, double *E, double *A, double *sdev);
+/* Fortran-layout FP32 results (interior written, ghosts untouched). */
+int dnsmc_get_results_fortran(dnsmc_solver *s, int snap, float *resfort, float *varfort);
+int dnsmc_get_stats(dnsmc_solver *s, dnsmc_stats *st);
+
+/* Test and training hooks. */
+/* Derived arrays: name in {"faces0","faces1",..,"centres","T1".."T5","cdf_band","cdf_g",
+ * "inv_emax","Ib","kq","Ibw","scalars"}; copies up to n floats, returns count. */
+int dnsmc_get_array(dnsmc_solver *s, const char *name, int level, float *out, long long n);
+/* Ray descriptors of the last run's first batch (SoA words) for parity tests. */
+long long dnsmc_get_rays(dnsmc_solver *s, unsigned int *w0, unsigned int *w1, float *w, float *winv,
+                         long long n);
+/* Per-ray record of the last run's first batch: steps, end type, hash, D, G. */
+long long dnsmc_get_ray_records(dnsmc_solver *s, int *steps, int *end_type, unsigned int *hash,
+                                float *D, float *G, long long n);
+/* Switch on per-ray records (costs bandwidth; off by default). */
+int dnsmc_set_record(dnsmc_solver *s, int on);
+/* Device synchronise (bench timing). */
+int dnsmc_sync(void);
+/* Measurement hook: `reps` launches of a plain device-to-device copy of
+ * `nbytes` (kernel copy_f32x4); returns read+write GB/s.  Used as the
+ * achievable-bandwidth reference and to calibrate HBM byte counters. */
+int dnsmc_bench_copy(long long nbytes, int reps, double *gbps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

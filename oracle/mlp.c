@@ -1,0 +1,111 @@
+/*
+ * FP64 reference of the proposal network (test infrastructure only).
+ *
+ * Network: X[nb x nin] -> H1 = tanh(X W1 + b1) -> H2 = tanh(H1 W2 + b2)
+ *          -> Z = H2 W3 + b3, Z = [direction logits (ndir) | bin logits].
+ * Loss (weighted maximum likelihood of the tallied bundles):
+ *   L = sum_s wt_s [-log softmax(Z_dir)[d_s] - log softmax(Z_bin)[b_s]] / sum_s wt_s
+ * The GPU forward/backward kernels are checked against these gradients, and
+ * these are checked against central finite differences of orc_mlp_loss_grad's
+ * loss (tests/test_oracle_mlp.py), so a dropped term or transposed operand
+ * fails.  No reference counterpart (the learned proposal is new).
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static void dense(int nb, int nin, int nout, const double *X, const double *W, const double *b,
+                  double *Y, int act)
+{
+    for (int s = 0; s < nb; ++s)
+        for (int o = 0; o < nout; ++o) {
+            double acc = b[o];
+            for (int i = 0; i < nin; ++i) acc += X[(size_t)s * nin + i] * W[(size_t)i * nout + o];
+            Y[(size_t)s * nout + o] = act ? tanh(acc) : acc;
+        }
+}
+
+void orc_mlp_forward(int nb, const double *X, const double *W1, const double *b1, const double *W2,
+                     const double *b2, const double *W3, const double *b3, int nin, int nh, int nout,
+                     double *H1, double *H2, double *Z)
+{
+    dense(nb, nin, nh, X, W1, b1, H1, 1);
+    dense(nb, nh, nh, H1, W2, b2, H2, 1);
+    dense(nb, nh, nout, H2, W3, b3, Z, 0);
+}
+
+static double softmax_ce(const double *z, int n, int t, double scale, double *dz)
+{
+    double m = z[0];
+    for (int i = 1; i < n; ++i) m = z[i] > m ? z[i] : m;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += exp(z[i] - m);
+    double lse = m + log(s);
+    for (int i = 0; i < n; ++i) dz[i] = scale * (exp(z[i] - lse) - (i == t ? 1.0 : 0.0));
+    return lse - z[t];
+}
+
+void orc_mlp_loss_grad(int nb, const double *X, const double *W1, const double *b1, const double *W2,
+                       const double *b2, const double *W3, const double *b3, int nin, int nh, int nout,
+                       int ndir, const int *td, const int *tb, const double *wt, double *loss,
+                       double *gW1, double *gb1, double *gW2, double *gb2, double *gW3, double *gb3)
+{
+    double *H1 = (double *)malloc(sizeof(double) * (size_t)nb * nh);
+    double *H2 = (double *)malloc(sizeof(double) * (size_t)nb * nh);
+    double *Z = (double *)malloc(sizeof(double) * (size_t)nb * nout);
+    double *dZ = (double *)malloc(sizeof(double) * (size_t)nb * nout);
+    double *dA2 = (double *)malloc(sizeof(double) * (size_t)nb * nh);
+    double *dA1 = (double *)malloc(sizeof(double) * (size_t)nb * nh);
+    orc_mlp_forward(nb, X, W1, b1, W2, b2, W3, b3, nin, nh, nout, H1, H2, Z);
+    double wsum = 0.0;
+    for (int s = 0; s < nb; ++s) wsum += wt[s];
+    double L = 0.0;
+    for (int s = 0; s < nb; ++s) {
+        double sc = wt[s] / wsum;
+        const double *z = Z + (size_t)s * nout;
+        double *dz = dZ + (size_t)s * nout;
+        L += sc * softmax_ce(z, ndir, td[s], sc, dz);
+        L += sc * softmax_ce(z + ndir, nout - ndir, tb[s], sc, dz + ndir);
+    }
+    *loss = L;
+    memset(gW3, 0, sizeof(double) * (size_t)nh * nout);
+    memset(gb3, 0, sizeof(double) * nout);
+    for (int s = 0; s < nb; ++s)
+        for (int o = 0; o < nout; ++o) {
+            double d = dZ[(size_t)s * nout + o];
+            gb3[o] += d;
+            for (int i = 0; i < nh; ++i) gW3[(size_t)i * nout + o] += H2[(size_t)s * nh + i] * d;
+        }
+    for (int s = 0; s < nb; ++s)
+        for (int i = 0; i < nh; ++i) {
+            double acc = 0.0;
+            for (int o = 0; o < nout; ++o) acc += dZ[(size_t)s * nout + o] * W3[(size_t)i * nout + o];
+            double h = H2[(size_t)s * nh + i];
+            dA2[(size_t)s * nh + i] = acc * (1.0 - h * h);
+        }
+    memset(gW2, 0, sizeof(double) * (size_t)nh * nh);
+    memset(gb2, 0, sizeof(double) * nh);
+    for (int s = 0; s < nb; ++s)
+        for (int o = 0; o < nh; ++o) {
+            double d = dA2[(size_t)s * nh + o];
+            gb2[o] += d;
+            for (int i = 0; i < nh; ++i) gW2[(size_t)i * nh + o] += H1[(size_t)s * nh + i] * d;
+        }
+    for (int s = 0; s < nb; ++s)
+        for (int i = 0; i < nh; ++i) {
+            double acc = 0.0;
+            for (int o = 0; o < nh; ++o) acc += dA2[(size_t)s * nh + o] * W2[(size_t)i * nh + o];
+            double h = H1[(size_t)s * nh + i];
+            dA1[(size_t)s * nh + i] = acc * (1.0 - h * h);
+        }
+    memset(gW1, 0, sizeof(double) * (size_t)nin * nh);
+    memset(gb1, 0, sizeof(double) * nh);
+    for (int s = 0; s < nb; ++s)
+        for (int o = 0; o < nh; ++o) {
+            double d = dA1[(size_t)s * nh + o];
+            gb1[o] += d;
+            for (int i = 0; i < nin; ++i) gW1[(size_t)i * nh + o] += X[(size_t)s * nin + i] * d;
+        }
+    free(H1); free(H2); free(Z); free(dZ); free(dA2); free(dA1);
+}

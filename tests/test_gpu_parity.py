@@ -1,0 +1,241 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.
+
+Bars: bit-exact for integers (band/g draws, direction uniform bits, per-ray
+step count, end type and visited-cell hash); FP32 per-ray values (D, G) within
+2e-5 relative (only exp rounding differs: v_exp_f32 vs glibc expf); FP64 cell
+tallies E, A within 1e-5 relative and Q = E - A within 1e-5 * E absolute.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import dnsmc_amd
+from dnsmc_amd import workloads
+from helpers import oracle_problem_dict, particle_table_dir, read_reference_tables, ref_subsequence
+
+pytestmark = pytest.mark.gpu
+
+
+def small_channel(n=(8, 12, 8), R=64, spectral=3, Tw=300.0, **kw):
+    return workloads.channel_config(n, rays_per_cell=R, spectral=spectral, Tw=Tw, **kw)
+
+
+def _decode(rays, nQ):
+    band = (rays["w0"] >> 24) | (((rays["w1"] >> 24) & 3) << 8)
+    g = rays["w1"] >> 26
+    return band.astype(np.int64), g.astype(np.int64)
+
+
+def _check_rays_and_tallies(oracle, S, cfg, gen, R):
+    d = oracle_problem_dict(S, cfg)
+    P = oracle.problem(d)
+    nemit = S.nemit
+    rays = S.rays(nemit * R)
+    rec = S.records(nemit * R)
+    assert len(rays) == nemit * R
+    res = S.results(0)
+    rel = []
+    for cell in range(nemit):
+        ref = gen(P, cell)
+        got = rays[cell * R:(cell + 1) * R]
+        np.testing.assert_array_equal(got["w0"], ref["w0"])
+        np.testing.assert_array_equal(got["w1"], ref["w1"])
+        np.testing.assert_allclose(got["w"], ref["w"], rtol=2e-7)
+        np.testing.assert_allclose(got["winv"], ref["winv"], rtol=2e-7)
+        i, j, k = cell % cfg.n[0] + 1, (cell // cfg.n[0]) % cfg.n[1] + 1, cell // (cfg.n[0] * cfg.n[1]) + 1
+        band, g = _decode(ref, d["nQ"])
+        for r in range(R):
+            ut = (float(ref["w0"][r] & 0xFFFFFF) + d["u_off"]) * 2.0 ** -24
+            up = (float(ref["w1"][r] & 0xFFFFFF) + d["u_off"]) * 2.0 ** -24
+            o = oracle.trace_ray(P, i, j, k, ut, up, int(band[r]), int(g[r]))
+            q = cell * R + r
+            assert rec["steps"][q] == o.steps, (cell, r)
+            assert rec["end"][q] == o.end_type, (cell, r)
+            assert rec["hash"][q] == o.hash, (cell, r)
+            assert abs(rec["D"][q] - o.D) <= 2e-5 * max(abs(o.D), 1e-30)
+            assert abs(rec["G"][q] - o.G) <= 2e-5 * max(abs(o.G), 1e-30)
+        E, A, Q2, _ = oracle.trace_descs(P, cell, ref)
+        assert abs(res["E"][cell] - E) <= 1e-5 * abs(E)
+        assert abs(res["A"][cell] - A) <= 1e-5 * abs(A)
+        assert abs(res["Q"][cell] - (E - A)) <= 1e-5 * abs(E)
+        rel.append(abs(res["Q"][cell] - (E - A)) / abs(E))
+    return float(np.max(rel))
+
+
+def test_grey_plain_parity(oracle):
+    cfg = small_channel(R=64)
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    T = workloads.field_grey32(3, tuple(cfg.n))
+    S.set_field(0, workloads.ghosted(T))
+    S.set_record(True)
+    S.run()
+    S.wait()
+    seed = cfg.seed
+    _check_rays_and_tallies(oracle, S, cfg, lambda P, c: oracle.gen_philox(P, seed, 0, c, 0, 64), 64)
+
+
+def test_ck_plain_parity(oracle, ck_table_dir):
+    tdir, t = ck_table_dir
+    cfg = small_channel(R=48, spectral=0, Tw=573.0, table_dir=tdir)
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    T = workloads.field_profile_noise(5, tuple(cfg.n))
+    S.set_field(0, workloads.ghosted(T))
+    S.set_record(True)
+    S.run()
+    S.wait()
+    # derived tables against the oracle's own construction (bit-exact)
+    Tmax = float(S.array("scalars")[0])
+    assert Tmax == max(float(T.max()), 573.0)
+    wq = workloads.gauss_legendre01(16)[1]
+    parsed = read_reference_tables(tdir, gas=True)
+    np.testing.assert_allclose(parsed["kq"], t["kq"], rtol=6e-7)   # writer round trip (%e: 7 digits)
+    ref = oracle.build_proposal(0, parsed, Tmax, 48, wq)
+    np.testing.assert_array_equal(S.array("Ib"), ref["Ib"])
+    np.testing.assert_array_equal(S.array("kq2")[0::2], ref["kq"])
+    np.testing.assert_array_equal(S.array("inv_emax"), ref["inv_emax"])
+    np.testing.assert_allclose(S.array("cdf_band"), ref["cdf_band"], rtol=0, atol=0)
+    np.testing.assert_allclose(S.array("cdf_g"), ref["cdf_g"], rtol=0, atol=0)
+    assert abs(S.Ibmax_d() - ref["Ibmax_d"]) <= 1e-12 * ref["Ibmax_d"]
+    seed = cfg.seed
+    _check_rays_and_tallies(oracle, S, cfg, lambda P, c: oracle.gen_philox(P, seed, 0, c, 0, 48), 48)
+
+
+def _compat_particle_cfg(n=(16, 16, 16), photmax=96, grid_num=3):
+    cfg = dnsmc_amd.default_config()
+    d = dict(n=n, photmax=photmax, nvar=3, grid_num=grid_num, maxs=(5, 5, 100000, 100000, 100000, 100000),
+             table_dir=particle_table_dir(), num_streams=16)
+    return dnsmc_amd.config_from_dict(d, cfg)
+
+
+def test_compat_particle_parity(oracle):
+    """Reference sampling (XORWOW per (cell, block), Sobol directions, nearest
+    node proposal) on the reference's own particle tables with 3 AMC levels."""
+    cfg = _compat_particle_cfg()
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    x = S.array("centres0", 0)[1:-1].astype(np.float64)
+    Tx = 500.0 - 2000.0 * x ** 2 + 2000.0 * x   # code/rad_main.f:68
+    T = np.broadcast_to(Tx[:, None, None], tuple(cfg.n)).astype(np.float32)
+    S.set_field(0, workloads.ghosted(np.ascontiguousarray(T)))
+    S.set_record(True)
+    S.run()
+    S.wait()
+    d = oracle_problem_dict(S, cfg)
+    # reference proposal rows, Planck table and Ibmax against the oracle
+    parsed = read_reference_tables(particle_table_dir(), gas=False)
+    sc = S.array("scalars")
+    ref = oracle.build_proposal(1, parsed, float(sc[0]), cfg.photmax, np.ones(1))
+    np.testing.assert_array_equal(S.array("Ib"), ref["Ib"])
+    np.testing.assert_array_equal(S.array("cdf_band"), ref["cdf_band"])
+    np.testing.assert_array_equal(S.array("cdf_g"), ref["cdf_g"])
+    assert float(sc[1]) == ref["kappamax"]
+    assert abs(S.Ibmax_d() - ref["Ibmax_d"]) <= 1e-15 * ref["Ibmax_d"]
+    # coarse levels from the GPU spline against the oracle's FP64 spline
+    f0 = [S.array("centres%d" % a, 0) for a in range(3)]
+    for l in range(1, cfg.grid_num):
+        fc = [S.array("centres%d" % a, l) for a in range(3)]
+        nc = [len(c) - 2 for c in fc]
+        ref = oracle.interp3d(list(cfg.n), f0[0], f0[1], f0[2], d["levels"][0]["T"], nc, fc[0], fc[1], fc[2])
+        np.testing.assert_allclose(d["levels"][l]["T"], ref, rtol=2e-7, atol=0)
+    P = oracle.problem(d)
+    R, nvar = cfg.photmax, cfg.nvar
+    rays = S.rays(S.nemit * nvar * R)
+    rec = S.records(S.nemit * nvar * R)
+    n = tuple(cfg.n)
+    cells = [0, 1, 17, 255, 1000, 2047, 4095]
+    for cell in cells:
+        i, j, k = cell % n[0], (cell // n[0]) % n[1], cell // (n[0] * n[1])
+        for v in range(nvar):
+            sub = ref_subsequence(i, j, k, v, n)
+            ref = oracle.gen_compat(P, cfg.seed, sub, 1, v * R, cell, R)
+            q0 = (cell * nvar + v) * R
+            got = rays[q0:q0 + R]
+            np.testing.assert_array_equal(got["w0"], ref["w0"])
+            np.testing.assert_array_equal(got["w1"], ref["w1"])
+            np.testing.assert_allclose(got["w"], ref["w"], rtol=2e-7)
+            band, g = _decode(ref, d["nQ"])
+            for r in range(0, R, 7):
+                ut = (float(ref["w0"][r] & 0xFFFFFF) + 1.0) * 2.0 ** -24
+                up = (float(ref["w1"][r] & 0xFFFFFF) + 1.0) * 2.0 ** -24
+                o = oracle.trace_ray(P, i + 1, j + 1, k + 1, ut, up, int(band[r]), int(g[r]))
+                assert rec["steps"][q0 + r] == o.steps
+                assert rec["end"][q0 + r] == o.end_type
+                assert rec["hash"][q0 + r] == o.hash
+
+
+def test_isothermal_enclosure_zero():
+    """Gas and walls at one temperature: Q == 0 to rounding (reciprocity)."""
+    cfg = small_channel(R=64, Tw=1000.0)
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    S.set_field(0, workloads.ghosted(np.full(tuple(cfg.n), 1000.0, np.float32)))
+    S.run()
+    r = S.results(0)
+    assert np.all(np.abs(r["Q"]) <= 1e-5 * r["E"])
+
+
+def test_grey_slab_closed_form(oracle):
+    """Unperturbed isothermal grey slab: cell-centre OERM expectation is the
+    plane-parallel exchange with the estimator's cut-off (E2 form without it)."""
+    cfg = small_channel(n=(4, 32, 4), R=20000)
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    S.set_field(0, workloads.ghosted(np.full(tuple(cfg.n), 1000.0, np.float32)))
+    S.run()
+    r = S.results(0)
+    fy = S.array("faces1", 0).astype(np.float64)
+    ny = cfg.n[1]
+    kap = np.ones(ny)
+    Ib = np.full(ny, 1000.0 ** 4)
+    F = oracle.slab_exchange(fy, kap, Ib, 300.0 ** 4, 300.0 ** 4, tau_cut=-math.log(1e-3), npanel=4000)
+    Qref = 4.0 * 5.670373e-08 * 1000.0 ** 4 * F
+    Q = r["Q"].reshape(cfg.n[2], ny, cfg.n[0])
+    z = (Q - Qref[None, :, None]) / r["sdev"].reshape(Q.shape)
+    assert np.max(np.abs(z)) < 5.0
+    assert abs(np.mean(z)) < 0.5
+
+
+def test_legacy_abi_fortran_layout(tmp_path):
+    """mc_gpu_/get_results_ through the Fortran-layout boundary."""
+    conf = tmp_path / "rad.conf"
+    conf.write_text("n = 16, 16, 16\nphotmax = 64\ngrid_num = 2\nmaxs = 5, 100000\ntally = 1\n"
+                    "table_dir = %s\n" % particle_table_dir())
+    os.environ["DNSMC_CONFIG"] = str(conf)
+    cfg = dnsmc_amd.parse_config(conf.read_text())
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    x = S.array("centres0", 0)
+    Tg = np.empty((18, 18, 18), np.float32, order="F")
+    Tg[...] = (500.0 - 2000.0 * x ** 2 + 2000.0 * x)[:, None, None]
+    S.set_field(0, Tg)
+    S.run()
+    Qs, Vs = S.results_fortran(0)
+    Q = np.full((18, 18, 18), -7.0, np.float32, order="F")
+    V = np.full((18, 18, 18), -7.0, np.float32, order="F")
+    dnsmc_amd.mc_gpu(Tg, 1)
+    dnsmc_amd.get_results(Q, V)
+    np.testing.assert_array_equal(Q[1:-1, 1:-1, 1:-1], Qs[1:-1, 1:-1, 1:-1])
+    np.testing.assert_array_equal(V[1:-1, 1:-1, 1:-1], Vs[1:-1, 1:-1, 1:-1])
+    assert np.all(Q[0] == -7.0) and np.all(Q[:, -1] == -7.0)
+
+
+@pytest.mark.parametrize("tally", [1, 2, 3])
+def test_tally_modes_agree(tally):
+    cfg0 = small_channel(R=200)
+    cfgt = small_channel(R=200, tally=tally)
+    T = workloads.ghosted(workloads.field_grey32(4, tuple(cfg0.n)))
+    out = []
+    for c in (cfg0, cfgt):
+        S = dnsmc_amd.Solver(c)
+        S.load_tables()
+        S.set_field(0, T)
+        S.run()
+        out.append(S.results(0))
+    tol = {1: 1e-12, 2: 2e-6, 3: 1e-5}[tally]
+    np.testing.assert_allclose(out[1]["E"], out[0]["E"], rtol=tol)
+    np.testing.assert_allclose(out[1]["A"], out[0]["A"], rtol=tol)
