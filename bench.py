@@ -64,7 +64,7 @@ def oracle_problem(S, cfg, Tint):
                           maxs=cfg.maxs[0], centres=[S.array("centres%d" % a, 0) for a in range(3)])],
              bc=[[cfg.bc[a][s] for s in range(2)] for a in range(3)], polar_axis=cfg.polar_axis,
              tau_cut=float(sc[10]), spectral=1 if cfg.spectral == 3 else 0, nB=int(sc[3]), nQ=int(sc[4]),
-             nT=int(sc[5]), inv_dT=float(sc[6]), mT0=float(sc[7]), kq=S.array("kq2")[0::2].copy(),
+             nT=int(sc[5]), inv_dT=float(sc[6]), mT0=float(sc[7]), kq=S.array("kq").copy(),
              Ib=S.array("Ib"), Ibw=S.array("Ibw"), kappa_grey=float(cfg.kappa_grey), cdf_band=S.array("cdf_band"),
              cdf_g=S.array("cdf_g"), inv_emax=S.array("inv_emax"), Ibmax=float(sc[2]), Ibmax_d=S.Ibmax_d(),
              Tmax=float(sc[0]), g_search=int(sc[8]), u_off=float(sc[9]))
@@ -136,6 +136,7 @@ def main():
         nrays += st["n_rays"]
         nsteps += st["n_steps"]
         launches += st["n_trace_launches"]
+        tables = ("global", "Ib in LDS", "kappa+Ib slice in LDS (%d nodes)" % st["table_nw"])[st["table_mode"]]
     dnsmc_amd.sync()
     t1 = time.perf_counter()
     dt = t1 - t0
@@ -160,7 +161,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "grid": list(cfg.n), "snapshots": cfg.nsnap, "rays_per_cell": R,
                    "table": "grey kappa=1" if table is None else "synthetic c-k 100x16x69",
-                   "estimator": "plain (Tmax proposal, Philox)", "tally": "fp64"},
+                   "estimator": "plain (Tmax proposal, Philox)", "tally": "fp64", "tables": tables},
         "segments_per_s": nsteps / dt,
         "segments_per_ray": nsteps / max(nrays, 1),
         "ms_trace_per_step": ms_trace / args.steps,

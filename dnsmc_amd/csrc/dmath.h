@@ -81,7 +81,8 @@ DN_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1)
     return c;
 }
 
-DN_HD float u01_open(uint32_t x) { return ((float)(x >> 8) + 0.5f) * kTwoM24; }
+// (m + 1/2) 2^-23 with m the top 23 bits: exact in FP32, strictly inside (0, 1).
+DN_HD float u01_open(uint32_t x) { return ((float)(x >> 9) + 0.5f) * 1.1920928955078125e-07f; }
 DN_HD float u01_incl(uint32_t x) { return ((float)(x >> 8) + 1.0f) * kTwoM24; }
 
 // XORWOW recurrence (Marsaglia 2003, "Xorshift RNGs", p.5).

@@ -435,7 +435,7 @@ int build_derived(const dnsmc_config& c, const Tables& t, float Tmax, Derived& d
     if (R < 1) { *err = "rays per cell must be positive"; return DNSMC_EINVAL; }
     if (c.spectral == DNSMC_SPEC_GREY_T4) {
         d.nB = 1; d.nQ = 1; d.nT = 2;
-        d.kq2.assign(4, 0.f);
+        d.kq.assign(2, 0.f);
         d.Ib.assign(2, 0.f);
         d.Ibw.resize(6);
         for (int a = 0; a < 3; ++a)
@@ -466,17 +466,12 @@ int build_derived(const dnsmc_config& c, const Tables& t, float Tmax, Derived& d
         *err = b;
         return DNSMC_ERANGE;
     }
-    // kappa rows as {k_t, k_{t+1} - k_t} pairs, row index (b*nQ + g)
-    d.kq2.assign((size_t)nB * nQ * nT * 2, 0.f);
+    // kappa rows, row index (b*nQ + g), temperature fastest
+    d.kq.assign((size_t)nB * nQ * nT, 0.f);
     for (int b = 0; b < nB; ++b)
         for (int g = 0; g < nQ; ++g)
-            for (int tt = 0; tt < nT; ++tt) {
-                float k0 = t.kq[((size_t)b * nT + tt) * nQ + g];
-                float k1 = tt + 1 < nT ? t.kq[((size_t)b * nT + tt + 1) * nQ + g] : k0;
-                size_t o = (((size_t)b * nQ + g) * nT + tt) * 2;
-                d.kq2[o] = k0;
-                d.kq2[o + 1] = k1 - k0;
-            }
+            for (int tt = 0; tt < nT; ++tt)
+                d.kq[((size_t)b * nQ + g) * nT + tt] = t.kq[((size_t)b * nT + tt) * nQ + g];
     // Planck table Ib[b][t] (code/mc_gpu.cu:541-551) and wall emission (black_copy)
     d.Ib.resize((size_t)nB * nT);
     for (int b = 0; b < nB; ++b)

@@ -14,11 +14,26 @@ struct LevParams {
     int n[3];
     int foff[3];      // offsets of the face arrays in the packed face buffer
     int maxs;
-    int pad;
-    long long tstride;   // cells of this level per snapshot
+    int nbx, nby;        // 4x4x4 bricks along x and y
+    long long tstride;   // floats of this level per snapshot (bricked, padded)
 };
 
+// 4x4x4-bricked field index of 0-based cell (x, y, z): bricks in (z, y, x)
+// order, 64 contiguous floats per brick.  A ray moving along any axis stays
+// inside the same 256-byte brick for 3 of 4 steps.
+__host__ __device__ __forceinline__ int brick_index(int x, int y, int z, int nbx, int nby)
+{
+    return ((((z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)) << 6) | ((z & 3) << 4) | ((y & 3) << 2) | (x & 3);
+}
+
 enum { kSpecTable = 0, kSpecGreyT4 = 1 };
+
+// Where the traversal reads its per-step table values from:
+//   kTabGlobal  kappa rows and Ib from global memory (L2),
+//   kTabIbLds   Ib table in LDS, kappa rows from global memory,
+//   kTabSliced  kappa rows and Ib of the temperature nodes the fields span
+//               ([tab_lo, tab_lo + tab_nw)) in LDS, 1024-thread blocks.
+enum { kTabGlobal = 0, kTabIbLds = 1, kTabSliced = 2 };
 
 struct TraceArgs {
     LevParams lev[6];
@@ -31,7 +46,7 @@ struct TraceArgs {
     int nB, nQ, nT;
     float inv_dT, mT0;
     float kappa_grey;
-    const float2* kq2;          // kappa rows {k, dk}
+    const float* kq;            // kappa rows [(b*nQ+g)*nT + t]
     const float* Ib;            // Planck table [b*nT + t]
     const float* Ibw;           // wall Planck [(axis*2+side)*nB + b]
     const float* faces;         // all levels packed
@@ -39,7 +54,7 @@ struct TraceArgs {
     const float* centres;       // level 0, x|y|z packed, ghost-inclusive
     int coff[3];
     int ncent;
-    int ib_in_lds;
+    int tab_lo, tab_nw;         // sliced tables: first node and node count held in LDS
     // rays (SoA)
     const uint32_t* w0;
     const uint32_t* w1;
@@ -49,10 +64,12 @@ struct TraceArgs {
     int chunk;
     int rays_per_slot;
     int slots_per_emitter;
-    FastDiv fd_R, fd_spe, fd_nx, fd_ny;   // divisions by rays/slot, slots/emitter, n0, n1
+    FastDiv fd_R;                         // division by rays per slot
     int refill_k;                         // start rays once this many lanes of a wave are idle
-    const int* emitters;        // emitter -> level-0 cell (i fastest, 0-based)
-    const int* order;           // processing position -> emitter (3-D brick order)
+    int super;                            // rays per wave super-chunk (multiple of 64)
+    const int4* slotinfo;       // processing slot -> {tally slot, cell, bricked index, i|j<<10|k<<20}
+    float2* pray;               // per-ray (E, A) for the deterministic tally modes
+    unsigned* claim;            // 8 super-chunk counters (one per XCD, 32 words apart), zero at launch
     long long slot0;            // first slot of this batch
     int snap;
     // tallies
@@ -60,8 +77,6 @@ struct TraceArgs {
     double* E;
     double* A;
     double* Q2;
-    double* part;               // per-chunk partials (deterministic modes), 3 per chunk
-    float* partf;               // FP32 per-chunk partials (Kahan / naive)
     unsigned long long* nsteps;
     // FP64 traversal extras
     const double* inv_emax_d;
@@ -84,7 +99,7 @@ struct GenArgs {
     float inv_dT, mT0;
     float kappa_grey;
     int spec;
-    const float2* kq2;
+    const float* kq;
     const float* Ib;
     const float* cdf_band;
     const float* cdf_g;
@@ -92,8 +107,7 @@ struct GenArgs {
     float Ibmax;
     int g_search;
     // batch
-    const int* emitters;
-    const int* order;
+    const int4* slotinfo;
     long long slot0;
     long long nslots;
     int rays_per_slot;
@@ -138,6 +152,8 @@ struct SplineArgs {
     const double* cA;
     const double* cB;
     long long snap_in, snap_out;   // per-snapshot strides
+    int brick_out, m0, nbx, nby;   // z pass: write the bricked layout of the coarse level
+    int in_bricked, fn1, fnbx, fnby;   // x pass: read the bricked fine field (line = j + n1 k)
 };
 
 }  // namespace dn

@@ -6,6 +6,8 @@
 //                  (band, g), Sobol32 dims 0/1 for directions.
 //   trace          persistent cell-by-cell traversal (code/mc_gpu.cu:228-438)
 //                  with FP64 per-slot tallies.
+//   reduce_rays    deterministic per-slot sum of per-ray tallies (FP64, Kahan
+//                  FP32 or plain FP32).
 //   spline_pass    one axis of the natural-spline coarse-grid interpolation
 //                  (code/spline.cpp:35-116).
 //   finalize       Q = E - A, standard error (code/mc_gpu.cu:212-222).
@@ -18,16 +20,7 @@
 
 namespace dn {
 
-// ----------------------------------------------------------- helpers ----
-__device__ __forceinline__ void lerp_row(const float* row, float pos, int nT, float* v)
-{
-    pos = fminf(fmaxf(pos, 0.0f), (float)(nT - 1));
-    int t0 = (int)pos;
-    t0 = t0 > nT - 2 ? nT - 2 : t0;
-    float fr = pos - (float)t0;
-    float a = row[t0], b = row[t0 + 1];
-    *v = fmaf(fr, b - a, a);
-}
+typedef __attribute__((address_space(1))) const float gfloat;
 
 // Emitter weights shared by the generators: Ibmax kappa_i Ib_i / (kappa_max Ib_max)
 // (code/mc_gpu.cu:63,:163-169).
@@ -46,8 +39,9 @@ __device__ __forceinline__ void bundle_weight(const GenArgs& a, const float* ib_
         int t0 = (int)pos;
         t0 = t0 > a.nT - 2 ? a.nT - 2 : t0;
         float fr = pos - (float)t0;
-        float2 k2 = a.kq2[(size_t)bg * a.nT + t0];
-        kap = fmaf(fr, k2.y, k2.x);
+        const float* kr = a.kq + (size_t)bg * a.nT;
+        float k0 = kr[t0], k1 = kr[t0 + 1];
+        kap = fmaf(fr, k1 - k0, k0);
         const float* ir = ib_tab + (size_t)band * a.nT;
         float i0 = ir[t0], i1 = ir[t0 + 1];
         ib = fmaf(fr, i1 - i0, i0);
@@ -77,13 +71,14 @@ __global__ __launch_bounds__(256) void raygen_plain(GenArgs a)
     for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nr; q += (long long)gridDim.x * blockDim.x) {
         long long sl = q / a.rays_per_slot;
         int r = (int)(q - sl * a.rays_per_slot);
-        int cell = a.emitters[a.order[(a.slot0 + sl) / a.slots_per_emitter]];
+        const int4 si = a.slotinfo[a.slot0 + sl];
+        const int cell = si.y;
         u32x4 x = philox4x32_10(u32x4{(uint32_t)r, (uint32_t)cell, (uint32_t)a.snap, 0u}, (uint32_t)a.seed,
                                 (uint32_t)(a.seed >> 32));
         int band = cdf_search(s_cdfb, a.nB, u01_open(x.x));
         int g = a.g_search ? cdf_search(s_cdfg + band * a.nQ, a.nQ, u01_open(x.y)) : 0;
         g = g > a.nQ - 1 ? a.nQ - 1 : g;
-        float Ti = a.T0[(size_t)a.snap * a.tstride0 + cell];
+        float Ti = a.T0[(size_t)a.snap * a.tstride0 + si.z];
         float w, winv;
         bundle_weight(a, ibt, Ti, band, g, &w, &winv);
         uint32_t w0, w1;
@@ -149,9 +144,9 @@ __global__ __launch_bounds__(256) void raygen_compat(GenArgs a)
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nthreads; t += (long long)gridDim.x * blockDim.x) {
         long long sl = t / runs;
         int run = (int)(t - sl * runs);
-        long long gslot = a.slot0 + sl;
-        int cell = a.emitters[a.order[gslot / a.slots_per_emitter]];
-        int v = (int)(gslot % a.slots_per_emitter);
+        const int4 si = a.slotinfo[a.slot0 + sl];
+        const int cell = si.y;
+        const int v = si.x % a.slots_per_emitter;
         // reference stream decomposition of the cell (code/mc_gpu.cu:84-87)
         int ci = cell % a.n0, cj = (cell / a.n0) % a.n1, ck = cell / (a.n0 * a.n1);
         int per_stream = a.n0 / a.num_streams;
@@ -175,7 +170,7 @@ __global__ __launch_bounds__(256) void raygen_compat(GenArgs a)
         int h1 = h0 + a.sub_len < a.rays_per_slot ? h0 + a.sub_len : a.rays_per_slot;
         uint64_t n = (uint64_t)v * (uint64_t)a.rays_per_slot + (uint64_t)h0;
         uint32_t sx0 = sobol_at(s_sob, n), sx1 = sobol_at(s_sob + 32, n);
-        float Ti = a.T0[(size_t)a.snap * a.tstride0 + cell];
+        float Ti = a.T0[(size_t)a.snap * a.tstride0 + si.z];
         for (int h = h0; h < h1; ++h) {
             uint32_t mp, mt;
             if (a.use_sobol) {
@@ -207,43 +202,75 @@ __global__ __launch_bounds__(256) void raygen_compat(GenArgs a)
 }
 
 // -------------------------------------------------------------- trace ----
-typedef __attribute__((address_space(1))) const float gfloat;
-typedef float f2v __attribute__((ext_vector_type(2)));
-typedef __attribute__((address_space(1))) const f2v gfloat2;
-
 __device__ __forceinline__ gfloat* lev_T(const int* L)
 {
     unsigned long long u = ((unsigned long long)(uint32_t)L[8] << 32) | (uint32_t)L[7];
     return (gfloat*)u;
 }
 
+// Tally one finished ray into the lane's running slot sums (TMODE 0) or the
+// per-ray buffer (TMODE 1).
+template <int TMODE>
+__device__ __forceinline__ void tally_ray(const TraceArgs& P, float er, float ar, int rslot, uint32_t q, int& aslot,
+                                          double& aE, double& aA, double& aQ)
+{
+    if (TMODE == 1) {
+        P.pray[q] = make_float2(er, ar);
+        return;
+    }
+    if (rslot != aslot) {
+        if (aslot >= 0) {
+            unsafeAtomicAdd(&P.E[aslot], aE);
+            unsafeAtomicAdd(&P.A[aslot], aA);
+            unsafeAtomicAdd(&P.Q2[aslot], aQ);
+        }
+        aslot = rslot;
+        aE = aA = aQ = 0.0;
+    }
+    const double de = (double)er, da = (double)ar, x = de - da;
+    aE += de;
+    aA += da;
+    aQ += x * x;
+}
+
 // Persistent cell-by-cell traversal (code/mc_gpu.cu:228-438).
 //
-// Work: the rays of the launch are cut into chunks of `chunk` rays of one
-// slot (emitting cell, or (cell, block) in compat mode); wave w takes chunks
-// w, w + W, w + 2W, ... (W = waves in the grid), so every wave sees the
-// whole launch and, at any time, the waves of the grid work on one compact
-// band of cells.  A lane keeps its chunk's tallies in registers and adds
-// them to the slot with one FP64 atomic each when the chunk ends.
+// Work: the launch's rays (in 3-D brick order of their emitting cells) are
+// cut into super-chunks of `super` consecutive rays.  Blocks are dispatched
+// round-robin over the 8 XCDs, so XCD x owns the contiguous super-chunk range
+// [x*N/8, (x+1)*N/8); its waves claim super-chunks of that range in order from
+// a per-XCD counter (one atomic per super-chunk), so the rays in flight on an
+// XCD stay inside a compact region its L2 holds and the 64 lanes of a wave
+// trace consecutive rays of the same one to three cells.  A wave whose XCD
+// range is used up claims from the next XCD's range, so no XCD idles while
+// another still has work.  The grid is sized to the resident capacity.
 //
 // Lanes whose ray ended stay idle until `refill_k` lanes of the wave are idle
-// (or none is alive); then one wave-uniform refill block tallies finished
-// rays, hands out chunks and starts rays for all idle lanes at once, so the
-// divergent start code runs once per batch instead of every step.
+// (or none is alive); one wave-uniform refill block then tallies the finished
+// rays and starts new rays on every idle lane (consecutive rays on
+// consecutive lanes: coalesced descriptor loads).
+//
+// Tallies: TMODE 0 keeps a running FP64 (E, A, sum x^2) per lane for its
+// current tally slot and adds it to the slot with three FP64 atomics when the
+// lane moves to a ray of another slot (consecutive rays share a slot, so one
+// atomic set covers several rays).  TMODE 1 writes each ray's (E, A) for the
+// deterministic reduce_rays pass.
 //
 // One step: the kappa gather of the current cell is issued first, the
 // geometry (distances, face snap, move, boundary) runs while it is in flight,
 // the next cell's temperature load is issued at the end of the geometry, and
 // the optical depth / transmissivity / tally update consumes kappa last.
-template <int SPEC, bool IB_LDS, bool F32TALLY, bool REC, bool AMC>
-__global__ __launch_bounds__(256) void trace(TraceArgs P)
+template <int SPEC, int TAB, int TMODE, bool REC, bool AMC>
+__global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArgs P)
 {
     extern __shared__ float lds[];
     int* s_lev = (int*)lds;                     // 6 * kLevWords
     float* s_face = lds + 6 * kLevWords;
-    float* s_cent = s_face + P.nfaces;
-    float* s_ibw = s_cent + P.ncent;            // 6 * nB wall Planck values
-    float* s_ib = s_ibw + 6 * P.nB;
+    float* s_ibw = s_face + P.nfaces;           // 6 * nB wall Planck values
+    float* s_ib = s_ibw + 6 * P.nB;             // Ib rows (nT or tab_nw nodes)
+    const int nw = P.tab_nw;
+    float* s_kq = s_ib + P.nB * nw;             // sliced kappa rows (kTabSliced)
+    gfloat* s_cent = (gfloat*)P.centres;        // cell centres: read at ray start only
     for (int i = threadIdx.x; i < P.nlev; i += blockDim.x) {
         int* L = s_lev + i * kLevWords;
         L[0] = P.lev[i].n[0]; L[1] = P.lev[i].n[1]; L[2] = P.lev[i].n[2];
@@ -253,82 +280,72 @@ __global__ __launch_bounds__(256) void trace(TraceArgs P)
         unsigned long long u = (unsigned long long)tb;
         L[7] = (int)(uint32_t)u;
         L[8] = (int)(uint32_t)(u >> 32);
+        L[9] = P.lev[i].nbx;
+        L[10] = P.lev[i].nby;
     }
     for (int i = threadIdx.x; i < P.nfaces; i += blockDim.x) s_face[i] = P.faces[i];
-    for (int i = threadIdx.x; i < P.ncent; i += blockDim.x) s_cent[i] = P.centres[i];
     for (int i = threadIdx.x; i < 6 * P.nB; i += blockDim.x) s_ibw[i] = P.Ibw[i];
-    if (SPEC == kSpecTable && IB_LDS)
+    if (SPEC == kSpecTable && TAB == kTabIbLds)
         for (int i = threadIdx.x; i < P.nB * P.nT; i += blockDim.x) s_ib[i] = P.Ib[i];
+    if (SPEC == kSpecTable && TAB == kTabSliced) {
+        for (int i = threadIdx.x; i < P.nB * nw; i += blockDim.x) {
+            const int r = i / nw;
+            s_ib[i] = P.Ib[r * P.nT + P.tab_lo + (i - r * nw)];
+        }
+        for (int i = threadIdx.x; i < P.nB * P.nQ * nw; i += blockDim.x) {
+            const int r = i / nw;
+            s_kq[i] = P.kq[r * P.nT + P.tab_lo + (i - r * nw)];
+        }
+    }
     __syncthreads();
-    const float* ibt = IB_LDS ? s_ib : P.Ib;
-    gfloat2* kq2 = (gfloat2*)P.kq2;
+    const float* ibt = TAB == kTabGlobal ? P.Ib : s_ib;
+    gfloat* kq = (gfloat*)P.kq;
+    const int rowlen = TAB == kTabSliced ? nw : P.nT;   // nodes per table row
 
     const int lane = threadIdx.x & 63;
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const uint32_t nrays = (uint32_t)P.nrays;
-    const uint32_t chunk = (uint32_t)P.chunk;
-    const uint32_t nchunks_all = (nrays + chunk - 1) / chunk;
-    // XCD-aware split: blocks are dispatched round-robin over the 8 XCDs, so
-    // XCD x (= b % 8) owns the contiguous chunk range [x*N/8, (x+1)*N/8) and
-    // its waves interleave inside it; each XCD's rays in flight then cover one
-    // compact region of cells that its own L2 holds.
+    const uint32_t S = (uint32_t)P.super;
+    const uint32_t nsc_all = (nrays + S - 1) / S;
     const bool split = (nb % 8u) == 0u;
     const uint32_t xcd = split ? b % 8u : 0u;
-    const uint32_t lb = split ? b / 8u : b;
-    const uint32_t wave = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const uint32_t nwaves = (split ? nb / 8u : nb) * (blockDim.x >> 6);
-    const uint32_t per_xcd = split ? (nchunks_all + 7u) / 8u : nchunks_all;
-    const uint32_t cbase = xcd * per_xcd;
-    const uint32_t nchunks = cbase >= nchunks_all ? 0u : (nchunks_all - cbase < per_xcd ? nchunks_all - cbase : per_xcd);
+    const uint32_t nparts = split ? 8u : 1u;
+    const uint32_t per_xcd = (nsc_all + nparts - 1u) / nparts;
     const int nT = P.nT;
-    const int nx0 = s_lev[0], ny0 = s_lev[1];
+    const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
 
-    uint32_t seq = 0;                 // wave-uniform chunk counter
-    uint32_t q = 0, qend = 0, qcur = 0, chunk_id = 0;
-    int slot = -1, ci = 1, cj = 1, ck = 1, idx_emit = 0;
-    bool alive = false, fin = false, more = true, have_next = false;
-    uint32_t nw0 = 0, nw1 = 0;
-    float nww = 0.f, nwi = 0.f;
+    // wave-uniform cursor over the claimed super-chunk
+    uint32_t xi = 0, cur = 0, cend = 0;
+    bool more = true;
+    // lane state
+    bool alive = false, fin = false;
+    uint32_t qcur = 0;
+    int rslot = -1, aslot = -1;
     double aE = 0.0, aA = 0.0, aQ = 0.0;
-    float fE = 0.f, fA = 0.f, fQ = 0.f, cE = 0.f, cA = 0.f, cQ = 0.f;
     uint32_t steps_lane = 0;
     int endt = 0;
     float Dfin = 1.0f;
-    // ray state
     float px = 0.f, py = 0.f, pz = 0.f, sx = 1.f, sy = 1.f, sz = 1.f, ix = 1.f, iy = 1.f, iz = 1.f;
     float fx = 0.f, fy = 0.f, fz = 0.f, tau = 0.f, tra = 1.f, G = 0.f, rw = 0.f, rwi = 0.f, Tc = 0.f;
     int cx = 1, cy = 1, cz = 1, fl = 0, krow = 0, ibrow = 0, band = 0, lev = 0, cnt = 0, idx = 0, rsteps = 0;
     uint32_t hash = 2166136261u;
     int n0 = s_lev[0], n1 = s_lev[1], n2 = s_lev[2];
     int fo0 = s_lev[3], fo1 = s_lev[4], fo2 = s_lev[5], maxs = s_lev[6];
+    int nbx = s_lev[9], nby = s_lev[10];
     gfloat* Tl = lev_T(s_lev);
     gfloat* T0 = Tl;
 
     for (;;) {
         const bool idle = !alive;
-        const bool has = (q < qend) || more;
-        const int ni = __popcll(__ballot(idle && has));
-        const int na = __popcll(__ballot(alive));
-        if (ni == 0 && na == 0) break;
-        if (ni >= P.refill_k || na == 0) {
-            // ---- tally rays finished since the last refill
+        const unsigned long long m_idle = __ballot(idle);
+        const int ni = __popcll(m_idle);
+        const int na = 64 - ni;
+        const bool have_rays = more || cur < cend;
+        if (na == 0 && !have_rays) break;
+        if (have_rays && (ni >= P.refill_k || na == 0)) {
+            // ---- tally the rays finished since the last refill
             if (fin) {
-                float er = rw * Dfin, ar = rwi * G;
-                if (F32TALLY) {
-                    float x = er - ar;
-                    if (P.tally == DNSMC_TALLY_KAHAN_F32) {
-                        float y = er - cE, t = fE + y; cE = (t - fE) - y; fE = t;
-                        y = ar - cA; t = fA + y; cA = (t - fA) - y; fA = t;
-                        y = x * x - cQ; t = fQ + y; cQ = (t - fQ) - y; fQ = t;
-                    } else {
-                        fE += er; fA += ar; fQ += x * x;
-                    }
-                } else {
-                    double de = (double)er, da = (double)ar, x = de - da;
-                    aE += de;
-                    aA += da;
-                    aQ += x * x;
-                }
+                tally_ray<TMODE>(P, rw * Dfin, rwi * G, rslot, qcur, aslot, aE, aA, aQ);
                 if (REC && qcur < P.nrec) {
                     P.rec_steps[qcur] = rsteps;
                     P.rec_end[qcur] = endt;
@@ -339,78 +356,59 @@ __global__ __launch_bounds__(256) void trace(TraceArgs P)
                 steps_lane += (uint32_t)rsteps;
                 fin = false;
             }
-            // ---- chunk hand-out (static, interleaved over waves; no atomics)
-            const bool need = idle && q >= qend && more;
-            if (need && slot >= 0) {
-                if (P.tally == DNSMC_TALLY_F64_ATOMIC) {
-                    unsafeAtomicAdd(&P.E[slot], aE);
-                    unsafeAtomicAdd(&P.A[slot], aA);
-                    unsafeAtomicAdd(&P.Q2[slot], aQ);
-                } else if (F32TALLY) {
-                    P.partf[3ull * chunk_id + 0] = fE;
-                    P.partf[3ull * chunk_id + 1] = fA;
-                    P.partf[3ull * chunk_id + 2] = fQ;
-                } else {
-                    P.part[3ull * chunk_id + 0] = aE;
-                    P.part[3ull * chunk_id + 1] = aA;
-                    P.part[3ull * chunk_id + 2] = aQ;
-                }
-                aE = aA = aQ = 0.0;
-                fE = fA = fQ = cE = cA = cQ = 0.f;
-                slot = -1;
-            }
-            const unsigned long long m = __ballot(need);
-            if (m) {
-                const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                const unsigned long long c = (unsigned long long)wave + (unsigned long long)(seq + rank) * nwaves;
-                seq += (uint32_t)__popcll(m);
-                if (need) {
-                    if (c < nchunks) {
-                        chunk_id = cbase + (uint32_t)c;
-                        q = chunk_id * chunk;
-                        qend = q + chunk < nrays ? q + chunk : nrays;
-                        // processing slot -> (emitter, block) -> tally slot
-                        const uint32_t ps = (uint32_t)P.slot0 + fdiv(q, P.fd_R);
-                        const uint32_t pe = fdiv(ps, P.fd_spe);
-                        const int e = P.order[pe];
-                        slot = e * P.slots_per_emitter + (int)(ps - pe * (uint32_t)P.slots_per_emitter);
-                        int cell = P.emitters[e];
-                        uint32_t t1 = fdiv((uint32_t)cell, P.fd_nx);
-                        uint32_t t2 = fdiv(t1, P.fd_ny);
-                        ci = cell - (int)t1 * nx0 + 1;
-                        cj = (int)t1 - (int)t2 * ny0 + 1;
-                        ck = (int)t2 + 1;
-                        idx_emit = cell;
-                        have_next = false;
-                    } else {
-                        more = false;
+            // ---- hand out consecutive rays to the idle lanes, in lane order
+            const uint32_t rank = (uint32_t)__popcll(m_idle & lanemask_lt);
+            const uint32_t avail = cend - cur;
+            uint32_t q = 0xFFFFFFFFu;
+            if (idle && rank < avail) q = cur + rank;
+            if (avail < (uint32_t)ni && more) {
+                // the current super-chunk runs out: claim the next one, from this
+                // XCD's range first, then from the following XCDs' ranges
+                uint32_t sc = 0xFFFFFFFFu;
+                while (xi < nparts) {
+                    const uint32_t X = (xcd + xi) % nparts;
+                    const uint32_t base = X * per_xcd;
+                    const uint32_t cnt = base >= nsc_all ? 0u : (nsc_all - base < per_xcd ? nsc_all - base : per_xcd);
+                    uint32_t c = 0;
+                    if (lane == 0) c = atomicAdd(P.claim + 32u * X, 1u);
+                    c = __builtin_amdgcn_readfirstlane(c);
+                    if (c < cnt) {
+                        sc = base + c;
+                        break;
                     }
+                    ++xi;
                 }
-            }
-            // ---- start rays on every idle lane that has one
-            if (idle && q < qend) {
-                qcur = q;
-                uint32_t a0, a1;
-                if (have_next) {
-                    a0 = nw0; a1 = nw1; rw = nww; rwi = nwi;
+                if (sc != 0xFFFFFFFFu) {
+                    const uint32_t nstart = sc * S;
+                    const uint32_t nend = nstart + S < nrays ? nstart + S : nrays;
+                    if (idle && rank >= avail && nstart + (rank - avail) < nend) q = nstart + (rank - avail);
+                    const uint32_t used = (uint32_t)ni - avail;
+                    cur = nstart + used < nend ? nstart + used : nend;
+                    cend = nend;
                 } else {
-                    a0 = P.w0[q]; a1 = P.w1[q]; rw = P.w[q]; rwi = P.winv[q];
+                    more = false;
+                    cur = cend;
                 }
-                ++q;
-                have_next = q < qend;
-                if (have_next) {   // prefetch the next descriptor of the chunk
-                    nw0 = P.w0[q]; nw1 = P.w1[q]; nww = P.w[q]; nwi = P.winv[q];
-                }
-                Tc = T0[idx_emit];
+            } else {
+                cur += avail < (uint32_t)ni ? avail : (uint32_t)ni;
+            }
+            // ---- start the handed-out rays
+            if (q != 0xFFFFFFFFu) {
+                qcur = q;
+                const uint32_t a0 = P.w0[q], a1 = P.w1[q];
+                rw = P.w[q];
+                rwi = P.winv[q];
+                const int4 si = P.slotinfo[P.slot0 + fdiv(q, P.fd_R)];
+                rslot = si.x;
                 band = desc_band(a0, a1);
-                int g = desc_g(a1);
-                float ut = ((float)(a0 & 0xFFFFFFu) + P.u_off) * kTwoM24;
-                float up = ((float)(a1 & 0xFFFFFFu) + P.u_off) * kTwoM24;
-                float cth = 1.0f - 2.0f * ut;
-                float sth = 2.0f * sqrtf(ut * (1.0f - ut));
+                const int g = desc_g(a1);
+                const float ut = ((float)(a0 & 0xFFFFFFu) + P.u_off) * kTwoM24;
+                const float up = ((float)(a1 & 0xFFFFFFu) + P.u_off) * kTwoM24;
+                const float cth = 1.0f - 2.0f * ut;
+                const float sth = 2.0f * sqrtf(ut * (1.0f - ut));
                 float sp, cp;
                 sincos2pi(up, &sp, &cp);
-                float s1 = sth * cp, s2 = sth * sp;
+                const float s1 = sth * cp, s2 = sth * sp;
                 sx = P.polar == 0 ? cth : (P.polar == 1 ? s2 : s1);
                 sy = P.polar == 0 ? s1 : (P.polar == 1 ? cth : s2);
                 sz = P.polar == 0 ? s2 : (P.polar == 1 ? s1 : cth);
@@ -421,26 +419,30 @@ __global__ __launch_bounds__(256) void trace(TraceArgs P)
                 ix = 1.0f / sx;
                 iy = 1.0f / sy;
                 iz = 1.0f / sz;
-                cx = ci; cy = cj; cz = ck;
-                px = s_cent[P.coff[0] + ci];
-                py = s_cent[P.coff[1] + cj];
-                pz = s_cent[P.coff[2] + ck];
+                cx = (si.w & 1023) + 1;
+                cy = ((si.w >> 10) & 1023) + 1;
+                cz = ((si.w >> 20) & 1023) + 1;
+                px = s_cent[P.coff[0] + cx];
+                py = s_cent[P.coff[1] + cy];
+                pz = s_cent[P.coff[2] + cz];
                 if (AMC && lev != 0) {
                     lev = 0;
                     n0 = s_lev[0]; n1 = s_lev[1]; n2 = s_lev[2];
                     fo0 = s_lev[3]; fo1 = s_lev[4]; fo2 = s_lev[5]; maxs = s_lev[6];
+                    nbx = s_lev[9]; nby = s_lev[10];
                     Tl = T0;
                 }
                 fx = s_face[fo0 + cx - (fl & 1)];
                 fy = s_face[fo1 + cy - ((fl >> 1) & 1)];
                 fz = s_face[fo2 + cz - ((fl >> 2) & 1)];
+                idx = si.z;
+                Tc = T0[idx];
                 tau = 0.0f;
                 tra = 1.0f;
                 G = 0.0f;
-                krow = (band * P.nQ + g) * nT;
-                ibrow = band * nT;
+                krow = (band * P.nQ + g) * rowlen;
+                ibrow = band * rowlen;
                 cnt = 0;
-                idx = idx_emit;
                 hash = 2166136261u;
                 rsteps = 0;
                 alive = true;
@@ -449,22 +451,36 @@ __global__ __launch_bounds__(256) void trace(TraceArgs P)
         if (!alive) continue;
 
         // ---------------------------------------------------- one crossing
-        f2v k2 = {0.f, 0.f};
-        float i0 = 0.f, i1 = 0.f, fr = 0.f;
+        float k0 = 0.f, k1 = 0.f, i0 = 0.f, i1 = 0.f, fr = 0.f;
         if (SPEC == kSpecTable) {
             float pos = fmaf(Tc, P.inv_dT, P.mT0);
             pos = fminf(fmaxf(pos, 0.0f), (float)(nT - 1));
             int t0 = (int)pos;
             t0 = t0 > nT - 2 ? nT - 2 : t0;
             fr = pos - (float)t0;
-            k2 = kq2[krow + t0];
-            i0 = ibt[ibrow + t0];
-            i1 = ibt[ibrow + t0 + 1];
+            if (TAB == kTabSliced) {
+                // node t0 of the slice; the slice spans every temperature of the
+                // fields, the clamp only keeps a corrupt field inside the table
+                int w = t0 - P.tab_lo;
+                w = w < 0 ? 0 : (w > nw - 2 ? nw - 2 : w);
+                k0 = s_kq[krow + w];
+                k1 = s_kq[krow + w + 1];
+                i0 = s_ib[ibrow + w];
+                i1 = s_ib[ibrow + w + 1];
+            } else {
+                k0 = kq[krow + t0];
+                k1 = kq[krow + t0 + 1];
+                i0 = ibt[ibrow + t0];
+                i1 = ibt[ibrow + t0 + 1];
+            }
         }
-        if (REC) hash = (hash ^ (((uint32_t)lev << 28) ^ (uint32_t)idx)) * 16777619u;
-        float d0 = (fx - px) * ix;
-        float d1 = (fy - py) * iy;
-        float d2 = (fz - pz) * iz;
+        if (REC) {
+            const int lin = (cx - 1) + n0 * ((cy - 1) + n1 * (cz - 1));
+            hash = (hash ^ (((uint32_t)lev << 28) ^ (uint32_t)lin)) * 16777619u;
+        }
+        const float d0 = (fx - px) * ix;
+        const float d1 = (fy - py) * iy;
+        const float d2 = (fz - pz) * iz;
         float ds = d0;
         ds = (ds > d1) ? d1 : ds;
         ds = (ds > d2) ? d2 : ds;
@@ -472,40 +488,40 @@ __global__ __launch_bounds__(256) void trace(TraceArgs P)
         py = (ds == d1) ? fy : fmaf(ds, sy, py);
         pz = (ds == d2) ? fz : fmaf(ds, sz, pz);
         const int ax = (d0 < d1) ? ((d0 < d2) ? 0 : 2) : ((d1 < d2) ? 1 : 2);
-        const bool m0 = ax == 0, m1 = ax == 1;
+        const bool m0 = ax == 0, m1 = ax == 1, m2 = ax == 2;
         const int fla = (fl >> ax) & 1;
-        int cm = m0 ? cx : (m1 ? cy : cz);
+        const int stp = fla ? -1 : 1;
+        int cm = (m0 ? cx : (m1 ? cy : cz)) + stp;
         const int nm = m0 ? n0 : (m1 ? n1 : n2);
         const int fo = m0 ? fo0 : (m1 ? fo1 : fo2);
-        cm += fla ? -1 : 1;
         const bool out = (cm == 0) | (cm == nm + 1);
         const int side = cm != 0;
         const int bcbit = ax * 2 + side;
         const bool per = (P.bc_periodic >> bcbit) & 1;
         const int wall = (out && !per) ? bcbit : -1;
         // periodic wrap; a wall hit keeps the old cell (the ray ends there)
-        const float pw = s_face[fo + (side ? 0 : nm)];
         const bool wrap = out && per;
-        cm = wrap ? (side ? 1 : nm) : (out ? cm - (fla ? -1 : 1) : cm);
+        const float pw = s_face[fo + (side ? 0 : nm)];
+        cm = wrap ? (side ? 1 : nm) : (out ? cm - stp : cm);
         px = (wrap && m0) ? pw : px;
         py = (wrap && m1) ? pw : py;
-        pz = (wrap && !m0 && !m1) ? pw : pz;
+        pz = (wrap && m2) ? pw : pz;
         const float fnew = s_face[fo + cm - fla];
         cx = m0 ? cm : cx;
         cy = m1 ? cm : cy;
-        cz = (!m0 && !m1) ? cm : cz;
+        cz = m2 ? cm : cz;
         fx = m0 ? fnew : fx;
         fy = m1 ? fnew : fy;
-        fz = (!m0 && !m1) ? fnew : fz;
-        idx = (cx - 1) + n0 * ((cy - 1) + n1 * (cz - 1));
+        fz = m2 ? fnew : fz;
+        idx = brick_index(cx - 1, cy - 1, cz - 1, nbx, nby);
         const float Tn = Tl[idx];
         float kap, ibj;
         if (SPEC == kSpecGreyT4) {
-            float t2 = Tc * Tc;
+            const float t2 = Tc * Tc;
             kap = P.kappa_grey;
             ibj = t2 * t2;
         } else {
-            kap = fmaf(fr, k2.y, k2.x);
+            kap = fmaf(fr, k1 - k0, k0);
             ibj = fmaf(fr, i1 - i0, i0);
         }
         ++cnt;
@@ -531,6 +547,7 @@ __global__ __launch_bounds__(256) void trace(TraceArgs P)
                     const int* L = s_lev + lev * kLevWords;
                     n0 = L[0]; n1 = L[1]; n2 = L[2];
                     fo0 = L[3]; fo1 = L[4]; fo2 = L[5]; maxs = L[6];
+                    nbx = L[9]; nby = L[10];
                     Tl = lev_T(L);
                     const int q0 = o0 / n0, q1 = o1 / n1, q2 = o2 / n2;
                     cx = (cx + q0 - 1) / q0;
@@ -539,7 +556,7 @@ __global__ __launch_bounds__(256) void trace(TraceArgs P)
                     fx = s_face[fo0 + cx - (fl & 1)];
                     fy = s_face[fo1 + cy - ((fl >> 1) & 1)];
                     fz = s_face[fo2 + cz - ((fl >> 2) & 1)];
-                    idx = (cx - 1) + n0 * ((cy - 1) + n1 * (cz - 1));
+                    idx = brick_index(cx - 1, cy - 1, cz - 1, nbx, nby);
                     Tc = Tl[idx];
                     cnt = 0;
                 } else {
@@ -548,23 +565,22 @@ __global__ __launch_bounds__(256) void trace(TraceArgs P)
             }
         }
     }
-    if (fin) {   // (cannot happen: the last refill tallies every finished ray)
+    // a ray that ended after the last refill (nothing was left to hand out)
+    if (fin) {
+        tally_ray<TMODE>(P, rw * Dfin, rwi * G, rslot, qcur, aslot, aE, aA, aQ);
+        if (REC && qcur < P.nrec) {
+            P.rec_steps[qcur] = rsteps;
+            P.rec_end[qcur] = endt;
+            P.rec_hash[qcur] = hash;
+            P.rec_D[qcur] = Dfin;
+            P.rec_G[qcur] = G;
+        }
         steps_lane += (uint32_t)rsteps;
     }
-    if (slot >= 0) {
-        if (P.tally == DNSMC_TALLY_F64_ATOMIC) {
-            unsafeAtomicAdd(&P.E[slot], aE);
-            unsafeAtomicAdd(&P.A[slot], aA);
-            unsafeAtomicAdd(&P.Q2[slot], aQ);
-        } else if (F32TALLY) {
-            P.partf[3ull * chunk_id + 0] = fE;
-            P.partf[3ull * chunk_id + 1] = fA;
-            P.partf[3ull * chunk_id + 2] = fQ;
-        } else {
-            P.part[3ull * chunk_id + 0] = aE;
-            P.part[3ull * chunk_id + 1] = aA;
-            P.part[3ull * chunk_id + 2] = aQ;
-        }
+    if (TMODE == 0 && aslot >= 0) {
+        unsafeAtomicAdd(&P.E[aslot], aE);
+        unsafeAtomicAdd(&P.A[aslot], aA);
+        unsafeAtomicAdd(&P.Q2[aslot], aQ);
     }
     // one step-count atomic per wave
     unsigned long long st = steps_lane;
@@ -572,48 +588,72 @@ __global__ __launch_bounds__(256) void trace(TraceArgs P)
     if (lane == 0) atomicAdd(P.nsteps, st);
 }
 
-#define DN_TRACE_INST(S, I, F, R, A) template __global__ void trace<S, I, F, R, A>(TraceArgs);
-#define DN_TRACE_INST2(S, I, F, R) DN_TRACE_INST(S, I, F, R, false) DN_TRACE_INST(S, I, F, R, true)
-DN_TRACE_INST2(kSpecTable, true, false, false)
-DN_TRACE_INST2(kSpecTable, true, false, true)
-DN_TRACE_INST2(kSpecTable, true, true, false)
-DN_TRACE_INST2(kSpecTable, false, false, false)
-DN_TRACE_INST2(kSpecTable, false, false, true)
-DN_TRACE_INST2(kSpecTable, false, true, false)
-DN_TRACE_INST2(kSpecGreyT4, false, false, false)
-DN_TRACE_INST2(kSpecGreyT4, false, false, true)
-DN_TRACE_INST2(kSpecGreyT4, false, true, false)
+#define DN_TRACE_INST(S, I, T, R, A) template __global__ void trace<S, I, T, R, A>(TraceArgs);
+#define DN_TRACE_INST2(S, I, T, R) DN_TRACE_INST(S, I, T, R, false) DN_TRACE_INST(S, I, T, R, true)
+#define DN_TRACE_INST3(S, I) DN_TRACE_INST2(S, I, 0, false) DN_TRACE_INST2(S, I, 0, true) DN_TRACE_INST2(S, I, 1, false)
+DN_TRACE_INST3(kSpecTable, kTabSliced)
+DN_TRACE_INST3(kSpecTable, kTabIbLds)
+DN_TRACE_INST3(kSpecTable, kTabGlobal)
+DN_TRACE_INST3(kSpecGreyT4, kTabGlobal)
 
-// ------------------------------------------------ deterministic reduce ----
-// Slot s owns chunks [s*cps, (s+1)*cps) of its batch; summed in a fixed order.
-__global__ void reduce_chunks(const double* part, const float* partf, int tally, long long nslots, int cps,
-                              long long slot0, double* E, double* A, double* Q2)
+// Range of a bricked level field as order-preserving keys of its FP32 values
+// (lohi = {min key, max key}; the host starts them at {~0u, 0}).
+__global__ void t_range(const float* T, int n0, int n1, int n2, int nbx, int nby, unsigned* lohi)
+{
+    const long long n = (long long)n0 * n1 * n2;
+    unsigned lo = 0xFFFFFFFFu, hi = 0u;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % n0), j = (int)((c / n0) % n1), k = (int)(c / ((long long)n0 * n1));
+        const unsigned u = __float_as_uint(T[brick_index(i, j, k, nbx, nby)]);
+        const unsigned key = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+        lo = key < lo ? key : lo;
+        hi = key > hi ? key : hi;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned a = __shfl_xor(lo, off), b = __shfl_xor(hi, off);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&lohi[0], lo);
+        atomicMax(&lohi[1], hi);
+    }
+}
+
+// ------------------------------------------------------ reduce_rays ----
+// Deterministic tallies: processing slot s owns rays [s*R, (s+1)*R) of the
+// batch; their (E, A) are summed in ray order in FP64, Kahan-compensated FP32
+// or plain FP32, and added to the slot's tally.
+__global__ void reduce_rays(const float2* pray, int tally, long long nslots, int R, long long slot0,
+                            const int4* slotinfo, double* E, double* A, double* Q2)
 {
     long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (s >= nslots) return;
+    const long long ts = slotinfo[slot0 + s].x;
+    const float2* p = pray + s * R;
     if (tally == DNSMC_TALLY_F64) {
         double e = 0, a = 0, q = 0;
-        for (int c = 0; c < cps; ++c) {
-            const double* p = part + 3 * (s * cps + c);
-            e += p[0]; a += p[1]; q += p[2];
+        for (int r = 0; r < R; ++r) {
+            double de = p[r].x, da = p[r].y, x = de - da;
+            e += de; a += da; q += x * x;
         }
-        E[slot0 + s] += e; A[slot0 + s] += a; Q2[slot0 + s] += q;
+        E[ts] += e; A[ts] += a; Q2[ts] += q;
     } else if (tally == DNSMC_TALLY_KAHAN_F32) {
         float e = 0, a = 0, q = 0, ce = 0, ca = 0, cq = 0;
-        for (int c = 0; c < cps; ++c) {
-            const float* p = partf + 3 * (s * cps + c);
-            float y = p[0] - ce, t = e + y; ce = (t - e) - y; e = t;
-            y = p[1] - ca; t = a + y; ca = (t - a) - y; a = t;
-            y = p[2] - cq; t = q + y; cq = (t - q) - y; q = t;
+        for (int r = 0; r < R; ++r) {
+            float x = p[r].x - p[r].y;
+            float y = p[r].x - ce, t = e + y; ce = (t - e) - y; e = t;
+            y = p[r].y - ca; t = a + y; ca = (t - a) - y; a = t;
+            y = x * x - cq; t = q + y; cq = (t - q) - y; q = t;
         }
-        E[slot0 + s] += e; A[slot0 + s] += a; Q2[slot0 + s] += q;
+        E[ts] += e; A[ts] += a; Q2[ts] += q;
     } else {
         float e = 0, a = 0, q = 0;
-        for (int c = 0; c < cps; ++c) {
-            const float* p = partf + 3 * (s * cps + c);
-            e += p[0]; a += p[1]; q += p[2];
+        for (int r = 0; r < R; ++r) {
+            float x = p[r].x - p[r].y;
+            e += p[r].x; a += p[r].y; q += x * x;
         }
-        E[slot0 + s] += e; A[slot0 + s] += a; Q2[slot0 + s] += q;
+        E[ts] += e; A[ts] += a; Q2[ts] += q;
     }
 }
 
@@ -651,33 +691,39 @@ __global__ void finalize(const double* E, const double* A, const double* Q2, lon
 // One thread per grid line: natural-spline second derivatives through the
 // fine values (forward sweep + back substitution, FP64), then evaluation at
 // the coarse centres.  The grid-only factors come precomputed from the host.
-__global__ void spline_pass(SplineArgs s, int snap)
+// The x pass reads the bricked fine field; the z pass writes the bricked
+// layout of the coarse level.
+__global__ void spline_pass(SplineArgs s)
 {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= s.nlines) return;
-    const float* in = s.in + snap * s.snap_in + (l % s.A) * s.sa_in + (l / s.A) * s.sb_in;
-    float* out = s.out + snap * s.snap_out + (l % s.A) * s.sa_out + (l / s.A) * s.sb_out;
     double* u = s.work + (size_t)l * s.nf;
     const int nf = s.nf;
-    // forward sweep: u[i] (y2 factors are grid-only)
+    const float* in = s.in + (l % s.A) * s.sa_in + (l / s.A) * s.sb_in;
+    const int bj = s.in_bricked ? l % s.fn1 : 0, bk = s.in_bricked ? l / s.fn1 : 0;
+    auto rd = [&](int i) -> double {
+        if (s.in_bricked) return (double)s.in[brick_index(i, bj, bk, s.fnbx, s.fnby)];
+        return (double)in[(long long)i * s.st_in];
+    };
     u[0] = 0.0;
-    double ym = in[0], y0 = in[s.st_in];
+    double ym = rd(0), y0 = rd(1);
     for (int i = 1; i < nf - 1; ++i) {
-        double yp = in[(long long)(i + 1) * s.st_in];
+        double yp = rd(i + 1);
         double dd = (yp - y0) * s.ihh[i] - (y0 - ym) * s.ihl[i];
         u[i] = (s.c6[i] * dd - s.sig[i] * u[i - 1]) * s.invp[i];
         ym = y0;
         y0 = yp;
     }
-    // back substitution in place: y2[nf-1] = 0, y2[k] = y2f[k] y2[k+1] + u[k]
     u[nf - 1] = 0.0;
     for (int k = nf - 2; k >= 1; --k) u[k] = s.y2f[k] * u[k + 1] + u[k];
     u[0] = 0.0;
     for (int q = 0; q < s.nc; ++q) {
         int k = s.klo[q];
-        double v = s.ca[q] * in[(long long)k * s.st_in] + s.cb[q] * in[(long long)(k + 1) * s.st_in] +
-                   s.cA[q] * u[k] + s.cB[q] * u[k + 1];
-        out[(long long)q * s.st_out] = (float)v;
+        double v = s.ca[q] * rd(k) + s.cb[q] * rd(k + 1) + s.cA[q] * u[k] + s.cB[q] * u[k + 1];
+        long long o;
+        if (s.brick_out) o = brick_index(l % s.m0, l / s.m0, q, s.nbx, s.nby);   // line l = i + m0 j
+        else o = (l % s.A) * s.sa_out + (l / s.A) * s.sb_out + (long long)q * s.st_out;
+        s.out[o] = (float)v;
     }
 }
 
@@ -687,6 +733,12 @@ __global__ void copy_f32x4(const float4* __restrict__ src, float4* __restrict__ 
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+__global__ void fill_u32(unsigned* p, long long n, unsigned v)
+{
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ void fill_f64(double* p, long long n, double v)

@@ -24,16 +24,18 @@
 
 namespace dn {
 // kernels.hip
-template <int SPEC, bool IB_LDS, bool F32TALLY, bool REC, bool AMC>
+template <int SPEC, int TAB, int TMODE, bool REC, bool AMC>
 __global__ void trace(TraceArgs P);
+__global__ void t_range(const float* T, int n0, int n1, int n2, int nbx, int nby, unsigned* lohi);
 __global__ void raygen_plain(GenArgs a);
 __global__ void raygen_compat(GenArgs a);
-__global__ void reduce_chunks(const double* part, const float* partf, int tally, long long nslots, int cps,
-                              long long slot0, double* E, double* A, double* Q2);
 __global__ void finalize(const double* E, const double* A, const double* Q2, long long nemit, int spe, int R,
                          int compat, double* Q, double* Eo, double* Ao, double* sd);
-__global__ void spline_pass(SplineArgs s, int snap);
+__global__ void spline_pass(SplineArgs s);
+__global__ void reduce_rays(const float2* pray, int tally, long long nslots, int R, long long slot0,
+                            const int4* slotinfo, double* E, double* A, double* Q2);
 __global__ void fill_f64(double* p, long long n, double v);
+__global__ void fill_u32(unsigned* p, long long n, unsigned v);
 __global__ void copy_f32x4(const float4* src, float4* dst, long long n4);
 }  // namespace dn
 
@@ -95,13 +97,18 @@ struct dnsmc_solver {
     Derived der;
     float der_Tmax = -1.f;
     int dev = 0, ncu = 256;
+    size_t lds_max = 65536;   // LDS bytes one block may allocate
     hipStream_t stream = nullptr;
 
     // fields
     std::vector<float> Tmax_snap;
+    std::vector<float> Tlo_snap, Thi_snap;   // range over all levels of each snapshot
     std::vector<char> field_set;
+    DevBuf<unsigned> d_lohi;
     DevBuf<float> d_T[kMaxLev];
-    long long ncell[kMaxLev] = {0, 0, 0, 0, 0, 0};
+    long long ncell[kMaxLev] = {0, 0, 0, 0, 0, 0};     // cells per level
+    long long tstride[kMaxLev] = {0, 0, 0, 0, 0, 0};   // bricked floats per level and snapshot
+    int nbx[kMaxLev] = {0, 0, 0, 0, 0, 0}, nby[kMaxLev] = {0, 0, 0, 0, 0, 0};
     std::vector<float> faces_packed, centres_packed;
     int foff[kMaxLev][3], coff[3];
     DevBuf<float> d_faces, d_centres;
@@ -114,7 +121,7 @@ struct dnsmc_solver {
     SplineArgs spa[kMaxLev][3];
     bool spline_ready = false;
     // tables
-    DevBuf<float> d_kq2, d_Ib, d_Ibw, d_cdfb, d_cdfg, d_inv_emax;
+    DevBuf<float> d_kq, d_Ib, d_Ibw, d_cdfb, d_cdfg, d_inv_emax;
     DevBuf<double> d_inv_emax_d;
     // compat
     DevBuf<uint32_t> d_xw_sub, d_xw_off, d_sobol;
@@ -123,6 +130,7 @@ struct dnsmc_solver {
     std::vector<int> emit;
     std::vector<int> order;     // processing position -> emitter, 3-D brick order
     DevBuf<int> d_emit, d_order;
+    DevBuf<int4> d_slotinfo;    // processing slot -> {tally slot, cell, bricked index, packed i|j|k}
     bool emit_dirty = true;
     // ray batches are pipelined over kPipes streams, each with its own buffers,
     // so the tail of one traversal launch overlaps the next batch
@@ -130,8 +138,8 @@ struct dnsmc_solver {
         hipStream_t stream = nullptr;
         DevBuf<uint32_t> d_w0, d_w1;
         DevBuf<float> d_w, d_winv;
-        DevBuf<double> d_part;
-        DevBuf<float> d_partf;
+        DevBuf<float2> d_pray;   // per-ray (E, A) for the deterministic tallies
+        DevBuf<unsigned> d_claim;   // per-XCD super-chunk counters of the traversal
         hipEvent_t done = nullptr;
     };
     static constexpr int kPipes = 3;
@@ -179,7 +187,8 @@ static int check_config(const dnsmc_config& c)
     if (c.nsnap < 1) return fail(DNSMC_EINVAL, "nsnap must be >= 1");
     if (c.tally < 0 || c.tally > 3) return fail(DNSMC_EINVAL, "unknown tally mode");
     if (c.precision != 0) return fail(DNSMC_EINVAL, "precision 1 (FP64 traversal) is not built in this library");
-    if (c.chunk < 1 || c.chunk > 4096) return fail(DNSMC_EINVAL, "chunk must be in [1, 4096]");
+    for (int a = 0; a < 3; ++a)
+        if (c.n[a] < 1 || c.n[a] > 1023) return fail(DNSMC_EINVAL, "grid sizes must be in [1, 1023]");
     if (c.batch_rays < 1024) return fail(DNSMC_EINVAL, "batch_rays must be >= 1024");
     if (c.spectral == DNSMC_SPEC_GREY_T4 && !(c.kappa_grey > 0.f)) return fail(DNSMC_EINVAL, "kappa_grey must be > 0");
     if (c.p_row < 1 || c.n[1] % c.p_row) return fail(DNSMC_EINVAL, "p_row must divide n[1]");
@@ -221,8 +230,12 @@ static int upload_geometry(dnsmc_solver* s)
     HIPCHK(s->d_faces.upload(s->faces_packed, s->stream));
     HIPCHK(s->d_centres.upload(s->centres_packed, s->stream));
     for (size_t l = 0; l < s->lev.size(); ++l) {
-        s->ncell[l] = (long long)s->lev[l].n[0] * s->lev[l].n[1] * s->lev[l].n[2];
-        HIPCHK(s->d_T[l].ensure((size_t)s->ncell[l] * s->cfg.nsnap));
+        const int* n = s->lev[l].n;
+        s->ncell[l] = (long long)n[0] * n[1] * n[2];
+        s->nbx[l] = (n[0] + 3) / 4;
+        s->nby[l] = (n[1] + 3) / 4;
+        s->tstride[l] = (long long)s->nbx[l] * s->nby[l] * ((n[2] + 3) / 4) * 64;
+        HIPCHK(s->d_T[l].ensure((size_t)s->tstride[l] * s->cfg.nsnap));
     }
     s->geom_uploaded = true;
     return DNSMC_OK;
@@ -293,12 +306,27 @@ static int run_spline(dnsmc_solver* s, int snap)
         for (int a = 0; a < 3; ++a) {
             SplineArgs A = s->spa[l][a];
             A.work = s->d_spw.p;
-            if (a == 0) { A.in = s->d_T[0].p + (size_t)snap * s->ncell[0]; A.out = tmpx.p; }
-            else if (a == 1) { A.in = tmpx.p; A.out = tmpy.p; }
-            else { A.in = tmpy.p; A.out = s->d_T[l].p + (size_t)snap * s->ncell[l]; }
+            if (a == 0) {   // bricked fine field in, line l = j + n1 k
+                A.in = s->d_T[0].p + (size_t)snap * s->tstride[0];
+                A.out = tmpx.p;
+                A.in_bricked = 1;
+                A.fn1 = L0.n[1];
+                A.fnbx = s->nbx[0];
+                A.fnby = s->nby[0];
+            } else if (a == 1) {
+                A.in = tmpx.p;
+                A.out = tmpy.p;
+            } else {        // bricked coarse level out, line l = i + m0 j
+                A.in = tmpy.p;
+                A.out = s->d_T[l].p + (size_t)snap * s->tstride[l];
+                A.brick_out = 1;
+                A.m0 = Lc.n[0];
+                A.nbx = s->nbx[l];
+                A.nby = s->nby[l];
+            }
             A.snap_in = A.snap_out = 0;
             int nb = (A.nlines + 127) / 128;
-            hipLaunchKernelGGL(spline_pass, dim3(nb), dim3(128), 0, s->stream, A, 0);
+            hipLaunchKernelGGL(spline_pass, dim3(nb), dim3(128), 0, s->stream, A);
             HIPCHK(hipGetLastError());
         }
     }
@@ -315,7 +343,7 @@ static int ensure_derived(dnsmc_solver* s, float Tmax)
     int rc = build_derived(s->cfg, s->tab, Tmax, s->der, &err);
     if (rc) return fail(rc, err);
     const Derived& d = s->der;
-    HIPCHK(s->d_kq2.upload(d.kq2, s->stream));
+    HIPCHK(s->d_kq.upload(d.kq, s->stream));
     HIPCHK(s->d_Ib.upload(d.Ib, s->stream));
     HIPCHK(s->d_Ibw.upload(d.Ibw, s->stream));
     HIPCHK(s->d_cdfb.upload(d.cdf_band, s->stream));
@@ -401,36 +429,56 @@ static void brick_order(const std::vector<int>& emit, int n0, int n1, int n2, st
     for (size_t p = 0; p < emit.size(); ++p) order[p] = key[p].second;
 }
 
-static int largest_divisor_leq(int R, int c)
+// Per processing slot: {tally slot, level-0 cell, bricked field index of the
+// cell, packed (i-1) | (j-1) << 10 | (k-1) << 20}.
+static void build_slotinfo(const std::vector<int>& emit, const std::vector<int>& order, int spe, const int* n,
+                           int nbx, int nby, std::vector<int4>& out)
 {
-    for (int d = std::min(R, c); d >= 1; --d)
-        if (R % d == 0) return d;
-    return 1;
-}
-
-template <int SPEC, bool IB, bool F32, bool REC>
-static void launch_trace_t(bool amc, dim3 g, dim3 b, size_t lds, hipStream_t st, const TraceArgs& P)
-{
-    if (amc) hipLaunchKernelGGL((trace<SPEC, IB, F32, REC, true>), g, b, lds, st, P);
-    else hipLaunchKernelGGL((trace<SPEC, IB, F32, REC, false>), g, b, lds, st, P);
-}
-
-static void launch_trace(int spec, bool ib, bool f32, bool rec, bool amc, dim3 g, dim3 b, size_t lds, hipStream_t st,
-                         const TraceArgs& P)
-{
-    if (spec == kSpecGreyT4) {
-        if (rec) launch_trace_t<kSpecGreyT4, false, false, true>(amc, g, b, lds, st, P);
-        else if (f32) launch_trace_t<kSpecGreyT4, false, true, false>(amc, g, b, lds, st, P);
-        else launch_trace_t<kSpecGreyT4, false, false, false>(amc, g, b, lds, st, P);
-    } else if (ib) {
-        if (rec) launch_trace_t<kSpecTable, true, false, true>(amc, g, b, lds, st, P);
-        else if (f32) launch_trace_t<kSpecTable, true, true, false>(amc, g, b, lds, st, P);
-        else launch_trace_t<kSpecTable, true, false, false>(amc, g, b, lds, st, P);
-    } else {
-        if (rec) launch_trace_t<kSpecTable, false, false, true>(amc, g, b, lds, st, P);
-        else if (f32) launch_trace_t<kSpecTable, false, true, false>(amc, g, b, lds, st, P);
-        else launch_trace_t<kSpecTable, false, false, false>(amc, g, b, lds, st, P);
+    out.resize(order.size() * (size_t)spe);
+    for (size_t pe = 0; pe < order.size(); ++pe) {
+        const int e = order[pe], cell = emit[e];
+        const int i = cell % n[0], j = (cell / n[0]) % n[1], k = cell / (n[0] * n[1]);
+        for (int v = 0; v < spe; ++v)
+            out[pe * spe + v] = make_int4(e * spe + v, cell, brick_index(i, j, k, nbx, nby), i | (j << 10) | (k << 20));
     }
+}
+
+// Persistent traversal launch: the grid is the resident capacity of the
+// device for this kernel variant (blocks per CU from the occupancy query,
+// times the CU count), capped by `max_blocks`, a multiple of 8 (one share per
+// XCD).  The sliced-table variant runs one 1024-thread block per CU.
+template <int SPEC, int TAB, int TM, bool REC>
+static hipError_t launch_trace_t(bool amc, int ncu, int max_blocks, size_t lds, hipStream_t st, const TraceArgs& P)
+{
+    void (*k)(TraceArgs) = amc ? trace<SPEC, TAB, TM, REC, true> : trace<SPEC, TAB, TM, REC, false>;
+    const int threads = TAB == kTabSliced ? 1024 : 256;
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    int nb = per_cu * ncu;
+    nb = std::min(nb, TAB == kTabSliced ? std::max(8, max_blocks / 4) : max_blocks);
+    nb = std::max(8, nb / 8 * 8);
+    hipLaunchKernelGGL(k, dim3(nb), dim3(threads), lds, st, P);
+    return hipGetLastError();
+}
+
+template <int SPEC, int TAB>
+static hipError_t launch_trace_m(bool det, bool rec, bool amc, int ncu, int max_blocks, size_t lds, hipStream_t st,
+                                 const TraceArgs& P)
+{
+    if (rec) return launch_trace_t<SPEC, TAB, 0, true>(amc, ncu, max_blocks, lds, st, P);
+    if (det) return launch_trace_t<SPEC, TAB, 1, false>(amc, ncu, max_blocks, lds, st, P);
+    return launch_trace_t<SPEC, TAB, 0, false>(amc, ncu, max_blocks, lds, st, P);
+}
+
+static hipError_t launch_trace(int spec, int tab, bool det, bool rec, bool amc, int ncu, int max_blocks, size_t lds,
+                               hipStream_t st, const TraceArgs& P)
+{
+    if (spec == kSpecGreyT4) return launch_trace_m<kSpecGreyT4, kTabGlobal>(det, rec, amc, ncu, max_blocks, lds, st, P);
+    if (tab == kTabSliced) return launch_trace_m<kSpecTable, kTabSliced>(det, rec, amc, ncu, max_blocks, lds, st, P);
+    if (tab == kTabIbLds) return launch_trace_m<kSpecTable, kTabIbLds>(det, rec, amc, ncu, max_blocks, lds, st, P);
+    return launch_trace_m<kSpecTable, kTabGlobal>(det, rec, amc, ncu, max_blocks, lds, st, P);
 }
 
 static int do_run(dnsmc_solver* s)
@@ -448,22 +496,25 @@ static int do_run(dnsmc_solver* s)
     rc = ensure_derived(s, Tmax);
     if (rc) return rc;
     if (c.mode == DNSMC_MODE_COMPAT && (rc = ensure_compat(s))) return rc;
+    const int spe = spe_of(c), R = rays_per_slot_of(c);
     if (s->emit_dirty) {
         brick_order(s->emit, c.n[0], c.n[1], c.n[2], s->order);
+        std::vector<int4> si;
+        build_slotinfo(s->emit, s->order, spe, c.n, s->nbx[0], s->nby[0], si);
         HIPCHK(s->d_emit.upload(s->emit, s->stream));
         HIPCHK(s->d_order.upload(s->order, s->stream));
+        HIPCHK(s->d_slotinfo.upload(si, s->stream));
         s->emit_dirty = false;
     }
     const Derived& d = s->der;
-    const int spe = spe_of(c), R = rays_per_slot_of(c);
     const long long nemit = (long long)s->emit.size();
     const long long nslots = nemit * spe;
-    const int chunk = largest_divisor_leq(R, c.chunk);
-    const long long slots_per_batch = std::max<long long>(1, std::min<long long>(c.batch_rays, 1ll << 30) / R);
-    const long long batch_rays = std::min(slots_per_batch, nslots) * R;
     const bool det = c.tally != DNSMC_TALLY_F64_ATOMIC;
-    const bool f32 = c.tally == DNSMC_TALLY_KAHAN_F32 || c.tally == DNSMC_TALLY_F32;
-    const long long nchunk_max = (batch_rays + chunk - 1) / chunk;
+    // ray batches: up to 2^30 rays (16 GB of descriptors) per launch; the
+    // deterministic tallies keep 8 more bytes per ray, so they use 2^27
+    const long long cap = det ? (1ll << 27) : (1ll << 30);
+    const long long slots_per_batch = std::max<long long>(1, std::min<long long>(c.batch_rays, cap) / R);
+    const long long batch_rays = std::min(slots_per_batch, nslots) * R;
     const long long nbatch_total = ((nslots + slots_per_batch - 1) / slots_per_batch) * c.nsnap;
     const int npipes = (int)std::min<long long>(dnsmc_solver::kPipes, nbatch_total);
     for (int p = 0; p < npipes; ++p) {
@@ -472,10 +523,8 @@ static int do_run(dnsmc_solver* s)
         HIPCHK(pp.d_w1.ensure(batch_rays));
         HIPCHK(pp.d_w.ensure(batch_rays));
         HIPCHK(pp.d_winv.ensure(batch_rays));
-        if (det) {
-            if (f32) HIPCHK(pp.d_partf.ensure(3 * nchunk_max));
-            else HIPCHK(pp.d_part.ensure(3 * nchunk_max));
-        }
+        if (det) HIPCHK(pp.d_pray.ensure(batch_rays));
+        HIPCHK(pp.d_claim.ensure(256));
     }
     HIPCHK(s->d_E.ensure(nslots * c.nsnap));
     HIPCHK(s->d_A.ensure(nslots * c.nsnap));
@@ -516,7 +565,9 @@ static int do_run(dnsmc_solver* s)
             P.lev[l].foff[a] = s->foff[l][a];
         }
         P.lev[l].maxs = s->lev[l].maxs;
-        P.lev[l].tstride = s->ncell[l];
+        P.lev[l].tstride = s->tstride[l];
+        P.lev[l].nbx = s->nbx[l];
+        P.lev[l].nby = s->nby[l];
         P.T[l] = s->d_T[l].p;
     }
     P.bc_periodic = 0;
@@ -529,7 +580,7 @@ static int do_run(dnsmc_solver* s)
     P.nB = d.nB; P.nQ = d.nQ; P.nT = d.nT;
     P.inv_dT = d.inv_dT; P.mT0 = d.mT0;
     P.kappa_grey = c.kappa_grey;
-    P.kq2 = (const float2*)s->d_kq2.p;
+    P.kq = s->d_kq.p;
     P.Ib = s->d_Ib.p;
     P.Ibw = s->d_Ibw.p;
     P.faces = s->d_faces.p;
@@ -538,23 +589,53 @@ static int do_run(dnsmc_solver* s)
     for (int a = 0; a < 3; ++a) P.coff[a] = s->coff[a];
     P.ncent = (int)s->centres_packed.size();
     const int spec = c.spectral == DNSMC_SPEC_GREY_T4 ? kSpecGreyT4 : kSpecTable;
-    const bool ib_lds = spec == kSpecTable && (size_t)d.nB * d.nT <= 12288;
-    P.ib_in_lds = ib_lds;
-    P.chunk = chunk;
+    // table placement: the kappa rows and Ib of the temperature nodes the
+    // fields span go to LDS when they fit one block's share; else Ib alone
+    // when small; else everything stays in global memory
+    int tab = kTabGlobal;
+    P.tab_lo = 0;
+    P.tab_nw = d.nT;
+    size_t lds_trace = sizeof(float) * (6 * kLevWords + P.nfaces + 6 * (size_t)d.nB);
+    if (spec == kSpecTable) {
+        float tlo = s->Tlo_snap[0], thi = s->Thi_snap[0];
+        for (int i = 1; i < c.nsnap; ++i) {
+            tlo = std::min(tlo, s->Tlo_snap[i]);
+            thi = std::max(thi, s->Thi_snap[i]);
+        }
+        auto node = [&](float T) {   // the kernel's t0 for temperature T
+            float pos = fmaf(T, d.inv_dT, d.mT0);
+            pos = std::min(std::max(pos, 0.0f), (float)(d.nT - 1));
+            int t0 = (int)pos;
+            return std::min(t0, d.nT - 2);
+        };
+        const int lo = node(tlo), hi = node(thi);
+        const int nw = hi + 2 - lo;
+        const size_t sliced = lds_trace + sizeof(float) * (size_t)d.nB * (d.nQ + 1) * nw;
+        const char* force = getenv("DNSMC_TABLES");   // "global": keep the tables out of LDS
+        const bool allow = !(force && !strcmp(force, "global"));
+        if (allow && sliced <= s->lds_max) {
+            tab = kTabSliced;
+            P.tab_lo = lo;
+            P.tab_nw = nw;
+            lds_trace = sliced;
+        } else if (allow && (size_t)d.nB * d.nT <= 12288) {
+            tab = kTabIbLds;
+            lds_trace += sizeof(float) * (size_t)d.nB * d.nT;
+        }
+    }
     P.rays_per_slot = R;
     P.slots_per_emitter = spe;
     P.fd_R = make_fastdiv((uint32_t)R);
-    P.fd_spe = make_fastdiv((uint32_t)spe);
-    P.fd_nx = make_fastdiv((uint32_t)c.n[0]);
-    P.fd_ny = make_fastdiv((uint32_t)c.n[1]);
     {
         const char* rk = getenv("DNSMC_REFILL_K");
         P.refill_k = rk ? atoi(rk) : 16;
         if (P.refill_k < 1) P.refill_k = 1;
         if (P.refill_k > 64) P.refill_k = 64;
+        const char* sc = getenv("DNSMC_SUPER");
+        P.super = sc ? atoi(sc) : 256;
+        P.super = std::max(64, (P.super + 63) / 64 * 64);
     }
-    P.emitters = s->d_emit.p;
-    P.order = s->d_order.p;
+    P.slotinfo = s->d_slotinfo.p;
     P.tally = c.tally;
     P.nsteps = s->d_nsteps.p;
     P.nrec = s->record ? s->nrec : 0;
@@ -562,27 +643,24 @@ static int do_run(dnsmc_solver* s)
     P.rec_D = s->d_rD.p; P.rec_G = s->d_rG.p;
     P.Tmax = d.Tmax;
     P.Ibmax_d = d.Ibmax_d;
-    const size_t lds_trace =
-        sizeof(float) * (6 * kLevWords + P.nfaces + P.ncent + 6 * (size_t)d.nB + (ib_lds ? (size_t)d.nB * d.nT : 0));
     const bool amc = P.nlev > 1;
 
     GenArgs G;
     memset(&G, 0, sizeof G);
     G.T0 = s->d_T[0].p;
-    G.tstride0 = s->ncell[0];
+    G.tstride0 = s->tstride[0];
     G.nB = d.nB; G.nQ = d.nQ; G.nT = d.nT;
     G.inv_dT = d.inv_dT; G.mT0 = d.mT0;
     G.kappa_grey = c.kappa_grey;
     G.spec = spec;
-    G.kq2 = (const float2*)s->d_kq2.p;
+    G.kq = s->d_kq.p;
     G.Ib = s->d_Ib.p;
     G.cdf_band = s->d_cdfb.p;
     G.cdf_g = s->d_cdfg.p;
     G.inv_emax = s->d_inv_emax.p;
     G.Ibmax = d.Ibmax;
     G.g_search = d.g_search;
-    G.emitters = s->d_emit.p;
-    G.order = s->d_order.p;
+    G.slotinfo = s->d_slotinfo.p;
     G.rays_per_slot = R;
     G.slots_per_emitter = spe;
     G.seed = c.seed;
@@ -601,7 +679,7 @@ static int do_run(dnsmc_solver* s)
     const bool gib = spec == kSpecTable && (size_t)d.nB * d.nT <= 16384;
     const size_t lds_gen = sizeof(float) * ((size_t)d.nB + (size_t)d.nB * d.nQ + (gib ? (size_t)d.nB * d.nT : 0) + 64);
 
-    const int trace_blocks = s->ncu * 8;
+    const int trace_blocks = s->ncu * 16;
     s->first_batch_rays = 0;
     long long bi = 0;
     for (int snap = 0; snap < c.nsnap; ++snap) {
@@ -628,8 +706,7 @@ static int do_run(dnsmc_solver* s)
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(eg.b, st));
             P.w0 = pp.d_w0.p; P.w1 = pp.d_w1.p; P.w = pp.d_w.p; P.winv = pp.d_winv.p;
-            P.part = pp.d_part.p;
-            P.partf = pp.d_partf.p;
+            P.pray = pp.d_pray.p;
             P.nrays = nr;
             P.slot0 = b0;
             P.snap = snap;
@@ -638,18 +715,21 @@ static int do_run(dnsmc_solver* s)
             P.Q2 = s->d_Q2.p + (size_t)snap * nslots;
             P.nrec = (s->record && snap == 0 && b0 == 0) ? s->nrec : 0;
             bool rec = P.nrec > 0;
-            long long nch = (nr + chunk - 1) / chunk;
-            int tb = (int)std::min<long long>(trace_blocks, std::max<long long>(1, (nch + 255) / 256));
+            // persistent grid, shrunk for small launches (about 4 super-chunks per wave)
+            const long long nsc = (nr + P.super - 1) / P.super;
+            const int tb_cap = (int)std::min<long long>(trace_blocks, std::max<long long>(8, (nsc + 15) / 16));
+            P.claim = pp.d_claim.p;
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, pp.d_claim.p, 256ll, 0u);
+            HIPCHK(hipGetLastError());
             EventPair& et = next_event(s, 1);
             HIPCHK(hipEventRecord(et.a, st));
-            launch_trace(spec, ib_lds, f32, rec, amc, dim3(tb), dim3(256), lds_trace, st, P);
-            HIPCHK(hipGetLastError());
+            HIPCHK(launch_trace(spec, tab, det, rec, amc, s->ncu, tb_cap, lds_trace, st, P));
             HIPCHK(hipEventRecord(et.b, st));
             s->st.n_trace_launches++;
-            if (det) {
+            if (det && !rec) {   // record launches tally with FP64 atomics
                 int nb = (int)((nsl + 255) / 256);
-                hipLaunchKernelGGL(reduce_chunks, dim3(nb), dim3(256), 0, st, (const double*)pp.d_part.p,
-                                   (const float*)pp.d_partf.p, c.tally, nsl, R / chunk, b0, P.E, P.A, P.Q2);
+                hipLaunchKernelGGL(reduce_rays, dim3(nb), dim3(256), 0, st, (const float2*)pp.d_pray.p, c.tally, nsl, R,
+                                   b0, (const int4*)s->d_slotinfo.p, P.E, P.A, P.Q2);
                 HIPCHK(hipGetLastError());
             }
         }
@@ -668,6 +748,9 @@ static int do_run(dnsmc_solver* s)
         HIPCHK(hipGetLastError());
     }
     s->st.n_rays = nslots * (long long)R * c.nsnap;
+    s->st.table_mode = tab;
+    s->st.table_lo = P.tab_lo;
+    s->st.table_nw = P.tab_nw;
     s->st.Tmax = d.Tmax;
     s->st.kappamax = d.kappamax;
     s->st.Ibmax = d.Ibmax_d;
@@ -752,6 +835,9 @@ int dnsmc_create(const dnsmc_config* cfg, dnsmc_solver** out)
     if (hipGetDevice(&s->dev) != hipSuccess) { delete s; return fail(DNSMC_EDEVICE, "no HIP device"); }
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->dev) == hipSuccess && ncu > 0) s->ncu = ncu;
+    int lmax = 0;
+    if (hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, s->dev) == hipSuccess && lmax > 0)
+        s->lds_max = (size_t)lmax;
     if (hipStreamCreate(&s->stream) != hipSuccess) { delete s; return fail(DNSMC_EDEVICE, "hipStreamCreate failed"); }
     s->pipe[0].stream = s->stream;
     for (int p = 1; p < dnsmc_solver::kPipes; ++p)
@@ -760,6 +846,8 @@ int dnsmc_create(const dnsmc_config* cfg, dnsmc_solver** out)
         if (hipEventCreateWithFlags(&s->pipe[p].done, hipEventDisableTiming) != hipSuccess) { dnsmc_destroy(s); return fail(DNSMC_EDEVICE, "hipEventCreate failed"); }
     if (hipEventCreateWithFlags(&s->ev_ready, hipEventDisableTiming) != hipSuccess) { dnsmc_destroy(s); return fail(DNSMC_EDEVICE, "hipEventCreate failed"); }
     s->Tmax_snap.assign(cfg->nsnap, 0.f);
+    s->Tlo_snap.assign(cfg->nsnap, 0.f);
+    s->Thi_snap.assign(cfg->nsnap, 0.f);
     s->field_set.assign(cfg->nsnap, 0);
     rc = upload_geometry(s);
     if (rc) { dnsmc_destroy(s); return rc; }
@@ -845,10 +933,11 @@ int dnsmc_set_field(dnsmc_solver* s, int snap, const float* T, int is_device)
         HIPCHK(hipMemcpy(host.data(), T, gsz * sizeof(float), hipMemcpyDeviceToHost));
         src = host.data();
     }
-    // interior (i fastest), Tmax, NaN check (code/mc_gpu.cu:456-480)
-    std::vector<float> in((size_t)n0 * n1 * n2);
+    // interior into the 4x4x4-bricked layout (padding cells hold 0 and are
+    // never read), Tmax, NaN check (code/mc_gpu.cu:456-480)
+    std::vector<float> in((size_t)s->tstride[0], 0.f);
+    const int nbx = s->nbx[0], nby = s->nby[0];
     float tmax = 0.f;
-    size_t o = 0;
     for (int k = 1; k <= n2; ++k)
         for (int j = 1; j <= n1; ++j) {
             const float* row = src + (size_t)(n0 + 2) * (j + (size_t)(n1 + 2) * k);
@@ -856,14 +945,42 @@ int dnsmc_set_field(dnsmc_solver* s, int snap, const float* T, int is_device)
                 float v = row[i];
                 if (!(v > 0.f) || !isfinite(v)) return fail(DNSMC_ERANGE, "temperature field holds a non-positive or non-finite value");
                 tmax = v > tmax ? v : tmax;
-                in[o++] = v;
+                in[brick_index(i - 1, j - 1, k - 1, nbx, nby)] = v;
             }
         }
-    HIPCHK(hipMemcpyAsync(s->d_T[0].p + (size_t)snap * s->ncell[0], in.data(), in.size() * sizeof(float),
+    HIPCHK(hipMemcpyAsync(s->d_T[0].p + (size_t)snap * s->tstride[0], in.data(), in.size() * sizeof(float),
                           hipMemcpyHostToDevice, s->stream));
     int rc = run_spline(s, snap);
     if (rc) return rc;
+    // temperature range over every level (the spline can leave the fine range)
+    HIPCHK(s->d_lohi.ensure(2 * s->lev.size()));
+    for (size_t l = 0; l < s->lev.size(); ++l) {
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s->stream, s->d_lohi.p + 2 * l, 1ll, 0xFFFFFFFFu);
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s->stream, s->d_lohi.p + 2 * l + 1, 1ll, 0u);
+        const int* ln = s->lev[l].n;
+        const long long ncl = (long long)ln[0] * ln[1] * ln[2];
+        const int nb = (int)std::min<long long>((ncl + 255) / 256, (long long)s->ncu * 16);
+        hipLaunchKernelGGL(t_range, dim3(nb), dim3(256), 0, s->stream, s->d_T[l].p + (size_t)snap * s->tstride[l],
+                           ln[0], ln[1], ln[2], s->nbx[l], s->nby[l], s->d_lohi.p + 2 * l);
+        HIPCHK(hipGetLastError());
+    }
+    std::vector<unsigned> lohi(2 * s->lev.size());
+    HIPCHK(hipMemcpyAsync(lohi.data(), s->d_lohi.p, lohi.size() * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    auto key_float = [](unsigned k) {
+        unsigned u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+        float f;
+        memcpy(&f, &u, sizeof f);
+        return f;
+    };
+    float lo = 0.f, hi = 0.f;
+    for (size_t l = 0; l < s->lev.size(); ++l) {
+        const float a = key_float(lohi[2 * l]), b = key_float(lohi[2 * l + 1]);
+        lo = l == 0 ? a : std::min(lo, a);
+        hi = l == 0 ? b : std::max(hi, b);
+    }
+    s->Tlo_snap[snap] = lo;
+    s->Thi_snap[snap] = hi;
     s->Tmax_snap[snap] = tmax;
     s->field_set[snap] = 1;
     return DNSMC_OK;
@@ -976,14 +1093,23 @@ int dnsmc_get_array(dnsmc_solver* s, const char* name, int level, float* out, lo
         v = s->lev[level].centres[nm[7] - '0'];
     } else if (nm == "T") {
         if (level < 0 || level >= (int)s->lev.size()) return fail(DNSMC_EINVAL, "level out of range");
+        // linear (i fastest) per snapshot, from the bricked device layout
+        std::vector<float> b((size_t)s->tstride[level] * s->cfg.nsnap);
+        HIPCHK(hipMemcpy(b.data(), s->d_T[level].p, b.size() * sizeof(float), hipMemcpyDeviceToHost));
+        const int* ln = s->lev[level].n;
         v.resize((size_t)s->ncell[level] * s->cfg.nsnap);
-        HIPCHK(hipMemcpy(v.data(), s->d_T[level].p, v.size() * sizeof(float), hipMemcpyDeviceToHost));
+        size_t o = 0;
+        for (int sn = 0; sn < s->cfg.nsnap; ++sn)
+            for (int k = 0; k < ln[2]; ++k)
+                for (int j = 0; j < ln[1]; ++j)
+                    for (int i = 0; i < ln[0]; ++i)
+                        v[o++] = b[(size_t)sn * s->tstride[level] + brick_index(i, j, k, s->nbx[level], s->nby[level])];
     } else if (nm == "cdf_band") v = s->der.cdf_band;
     else if (nm == "cdf_g") v = s->der.cdf_g;
     else if (nm == "inv_emax") v = s->der.inv_emax;
     else if (nm == "Ib") v = s->der.Ib;
     else if (nm == "Ibw") v = s->der.Ibw;
-    else if (nm == "kq2") v = s->der.kq2;
+    else if (nm == "kq") v = s->der.kq;
     else if (nm == "scalars") {
         const Derived& d = s->der;
         v = {d.Tmax, d.kappamax, d.Ibmax, (float)d.nB, (float)d.nQ, (float)d.nT, d.inv_dT, d.mT0,

@@ -31,7 +31,7 @@ struct Level {
 struct Derived {
     int nB = 1, nQ = 1, nT = 2;
     float inv_dT = 0.f, mT0 = 0.f;
-    std::vector<float> kq2;      // [(b*nQ+g)*nT + t] pairs {k_t, k_{t+1}-k_t} -> 2 floats
+    std::vector<float> kq;       // kappa rows [(b*nQ+g)*nT + t]
     std::vector<float> Ib;       // [b*nT + t]
     std::vector<float> Ibw;      // [(axis*2+side)*nB + b]
     std::vector<float> cdf_band; // nB

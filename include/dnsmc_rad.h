@@ -118,6 +118,8 @@ typedef struct dnsmc_stats {
     float Tmax;
     float kappamax;
     double Ibmax;
+    int table_mode;           /* per-step table reads: 0 global, 1 Ib in LDS, 2 kappa+Ib slice in LDS */
+    int table_lo, table_nw;   /* temperature nodes held in LDS (mode 2) */
 } dnsmc_stats;
 
 /* Defaults equal to the reference's shipped code/param.h + NarrowBand.h. */

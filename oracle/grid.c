@@ -230,6 +230,7 @@ void orc_slab_grey_Q(int ny, const double *faces, const double *T, double kappa,
                      double Tw_lo, double Tw_hi, double *Q)
 {
     const double sigma = 5.670373e-08;
+    if (ny < 1) return;
     double *kap = (double *)malloc(sizeof(double) * ny), *Ib = (double *)malloc(sizeof(double) * ny);
     for (int j = 0; j < ny; ++j) { kap[j] = kappa; Ib[j] = T[j] * T[j] * T[j] * T[j]; }
     orc_slab_exchange(ny, faces, kap, Ib, pow(Tw_lo, 4), pow(Tw_hi, 4), Q);

@@ -47,7 +47,8 @@ void orc_xorwow_seed(orc_xorwow *s, uint64_t seed);
 void orc_xorwow_skip_brute(orc_xorwow *s, uint64_t n);
 /* Jump by subsequence * 2^67 + offset using GF(2) matrix powers built here. */
 void orc_xorwow_init(orc_xorwow *s, uint64_t seed, uint64_t subsequence, uint64_t offset);
-/* 32-bit draw -> float uniform in (0,1] (compat modes) / (0,1) (Philox mode). */
+/* 32-bit draw -> float uniform: (m+1) 2^-24 on the top 24 bits, in (0,1] (compat
+ * modes); (m+1/2) 2^-23 on the top 23 bits, strictly in (0,1) (Philox mode). */
 float orc_u01_incl(uint32_t x);
 float orc_u01_open(uint32_t x);
 

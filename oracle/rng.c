@@ -49,7 +49,8 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 }
 
 float orc_u01_incl(uint32_t x) { return ((float)(x >> 8) + 1.0f) * 5.9604644775390625e-08f; }
-float orc_u01_open(uint32_t x) { return ((float)(x >> 8) + 0.5f) * 5.9604644775390625e-08f; }
+/* (m + 1/2) 2^-23, m = top 23 bits: exact in FP32 and strictly inside (0, 1). */
+float orc_u01_open(uint32_t x) { return ((float)(x >> 9) + 0.5f) * 1.1920928955078125e-07f; }
 
 /* ------------------------------------------------------------- XORWOW ---- */
 uint32_t orc_xorwow_next(orc_xorwow *s)

@@ -22,7 +22,7 @@ def oracle_problem_dict(S, cfg, snap=0):
         levels.append(lev)
     levels[0]["centres"] = [S.array("centres%d" % a, 0) for a in range(3)]
     sc = S.array("scalars")
-    kq2 = S.array("kq2")
+    kq1 = S.array("kq")
     return dict(
         levels=levels,
         bc=[[cfg.bc[a][s] for s in range(2)] for a in range(3)],
@@ -31,7 +31,7 @@ def oracle_problem_dict(S, cfg, snap=0):
         spectral=1 if cfg.spectral == 3 else 0,
         nB=int(sc[3]), nQ=int(sc[4]), nT=int(sc[5]),
         inv_dT=float(sc[6]), mT0=float(sc[7]),
-        kq=kq2[0::2].copy(),
+        kq=kq1.copy(),
         Ib=S.array("Ib"), Ibw=S.array("Ibw"),
         kappa_grey=float(cfg.kappa_grey),
         cdf_band=S.array("cdf_band"), cdf_g=S.array("cdf_g"), inv_emax=S.array("inv_emax"),
