@@ -7,7 +7,7 @@
 
 namespace dn {
 
-constexpr int kLevWords = 12;   // per-level parameter words staged in LDS
+constexpr int kLevWords = 16;   // per-level parameter words staged in LDS
 
 // Per-level geometry, staged into LDS as ints (floats for nothing).
 struct LevParams {
@@ -15,6 +15,7 @@ struct LevParams {
     int foff[3];      // offsets of the face arrays in the packed face buffer
     int maxs;
     int nbx, nby;        // 4x4x4 bricks along x and y
+    int ooff[3];         // offsets of the brick-offset tables in the packed table buffer
     long long tstride;   // floats of this level per snapshot (bricked, padded)
 };
 
@@ -51,6 +52,8 @@ struct TraceArgs {
     const float* Ibw;           // wall Planck [(axis*2+side)*nB + b]
     const float* faces;         // all levels packed
     int nfaces;
+    const int* offs;            // per level and axis: brick-layout offset of 0-based cell index c
+    int noffs;
     const float* centres;       // level 0, x|y|z packed, ghost-inclusive
     int coff[3];
     int ncent;

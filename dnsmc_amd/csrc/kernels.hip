@@ -208,6 +208,14 @@ __device__ __forceinline__ gfloat* lev_T(const int* L)
     return (gfloat*)u;
 }
 
+// Temperature of bricked cell index idx: a 32-bit byte offset from the level
+// base (fields stay below 4 GiB), so a uniform base addresses in one VGPR.
+__device__ __forceinline__ float load_T(gfloat* base, int idx)
+{
+    typedef __attribute__((address_space(1))) const char gchar;
+    return *(gfloat*)((gchar*)base + (uint32_t)idx * 4u);
+}
+
 // Tally one finished ray into the lane's running slot sums (TMODE 0) or the
 // per-ray buffer (TMODE 1).
 template <int TMODE>
@@ -266,11 +274,12 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     extern __shared__ float lds[];
     int* s_lev = (int*)lds;                     // 6 * kLevWords
     float* s_face = lds + 6 * kLevWords;
-    float* s_ibw = s_face + P.nfaces;           // 6 * nB wall Planck values
+    int* s_off = (int*)(s_face + P.nfaces);     // brick offsets per level and axis
+    float* s_cent = (float*)(s_off + P.noffs);  // level-0 cell centres (ghost-inclusive)
+    float* s_ibw = s_cent + P.ncent;            // 6 * nB wall Planck values
     float* s_ib = s_ibw + 6 * P.nB;             // Ib rows (nT or tab_nw nodes)
     const int nw = P.tab_nw;
     float* s_kq = s_ib + P.nB * nw;             // sliced kappa rows (kTabSliced)
-    gfloat* s_cent = (gfloat*)P.centres;        // cell centres: read at ray start only
     for (int i = threadIdx.x; i < P.nlev; i += blockDim.x) {
         int* L = s_lev + i * kLevWords;
         L[0] = P.lev[i].n[0]; L[1] = P.lev[i].n[1]; L[2] = P.lev[i].n[2];
@@ -282,8 +291,13 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         L[8] = (int)(uint32_t)(u >> 32);
         L[9] = P.lev[i].nbx;
         L[10] = P.lev[i].nby;
+        L[11] = P.lev[i].ooff[0];
+        L[12] = P.lev[i].ooff[1];
+        L[13] = P.lev[i].ooff[2];
     }
     for (int i = threadIdx.x; i < P.nfaces; i += blockDim.x) s_face[i] = P.faces[i];
+    for (int i = threadIdx.x; i < P.noffs; i += blockDim.x) s_off[i] = P.offs[i];
+    for (int i = threadIdx.x; i < P.ncent; i += blockDim.x) s_cent[i] = P.centres[i];
     for (int i = threadIdx.x; i < 6 * P.nB; i += blockDim.x) s_ibw[i] = P.Ibw[i];
     if (SPEC == kSpecTable && TAB == kTabIbLds)
         for (int i = threadIdx.x; i < P.nB * P.nT; i += blockDim.x) s_ib[i] = P.Ib[i];
@@ -331,7 +345,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     uint32_t hash = 2166136261u;
     int n0 = s_lev[0], n1 = s_lev[1], n2 = s_lev[2];
     int fo0 = s_lev[3], fo1 = s_lev[4], fo2 = s_lev[5], maxs = s_lev[6];
-    int nbx = s_lev[9], nby = s_lev[10];
+    int oo0 = s_lev[11], oo1 = s_lev[12], oo2 = s_lev[13];
     gfloat* Tl = lev_T(s_lev);
     gfloat* T0 = Tl;
 
@@ -429,7 +443,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                     lev = 0;
                     n0 = s_lev[0]; n1 = s_lev[1]; n2 = s_lev[2];
                     fo0 = s_lev[3]; fo1 = s_lev[4]; fo2 = s_lev[5]; maxs = s_lev[6];
-                    nbx = s_lev[9]; nby = s_lev[10];
+                    oo0 = s_lev[11]; oo1 = s_lev[12]; oo2 = s_lev[13];
                     Tl = T0;
                 }
                 fx = s_face[fo0 + cx - (fl & 1)];
@@ -461,8 +475,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
             if (TAB == kTabSliced) {
                 // node t0 of the slice; the slice spans every temperature of the
                 // fields, the clamp only keeps a corrupt field inside the table
-                int w = t0 - P.tab_lo;
-                w = w < 0 ? 0 : (w > nw - 2 ? nw - 2 : w);
+                const int w = max(0, min(t0 - P.tab_lo, nw - 2));
                 k0 = s_kq[krow + w];
                 k1 = s_kq[krow + w + 1];
                 i0 = s_ib[ibrow + w];
@@ -513,8 +526,8 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         fx = m0 ? fnew : fx;
         fy = m1 ? fnew : fy;
         fz = m2 ? fnew : fz;
-        idx = brick_index(cx - 1, cy - 1, cz - 1, nbx, nby);
-        const float Tn = Tl[idx];
+        idx = s_off[oo0 + cx - 1] + s_off[oo1 + cy - 1] + s_off[oo2 + cz - 1];
+        const float Tn = load_T(Tl, idx);
         float kap, ibj;
         if (SPEC == kSpecGreyT4) {
             const float t2 = Tc * Tc;
@@ -526,6 +539,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         }
         ++cnt;
         ++rsteps;
+        Tc = Tn;   // (read on every path, so the load is issued here, not sunk)
         const float tau1 = fmaf(kap, ds, tau);
         if (tau1 >= P.tau_cut) {
             G = fmaf(tra, ibj, G);
@@ -535,7 +549,6 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
             G = fmaf(tra - tra1, ibj, G);
             tau = tau1;
             tra = tra1;
-            Tc = Tn;
             if (wall >= 0) {
                 G = fmaf(tra, s_ibw[wall * P.nB + band], G);
                 alive = false; fin = true; endt = 1; Dfin = 1.0f;
@@ -547,7 +560,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                     const int* L = s_lev + lev * kLevWords;
                     n0 = L[0]; n1 = L[1]; n2 = L[2];
                     fo0 = L[3]; fo1 = L[4]; fo2 = L[5]; maxs = L[6];
-                    nbx = L[9]; nby = L[10];
+                    oo0 = L[11]; oo1 = L[12]; oo2 = L[13];
                     Tl = lev_T(L);
                     const int q0 = o0 / n0, q1 = o1 / n1, q2 = o2 / n2;
                     cx = (cx + q0 - 1) / q0;
@@ -556,8 +569,8 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                     fx = s_face[fo0 + cx - (fl & 1)];
                     fy = s_face[fo1 + cy - ((fl >> 1) & 1)];
                     fz = s_face[fo2 + cz - ((fl >> 2) & 1)];
-                    idx = brick_index(cx - 1, cy - 1, cz - 1, nbx, nby);
-                    Tc = Tl[idx];
+                    idx = s_off[oo0 + cx - 1] + s_off[oo1 + cy - 1] + s_off[oo2 + cz - 1];
+                    Tc = load_T(Tl, idx);
                     cnt = 0;
                 } else {
                     alive = false; fin = true; endt = 2; Dfin = 1.0f - tra;

@@ -110,7 +110,9 @@ struct dnsmc_solver {
     long long tstride[kMaxLev] = {0, 0, 0, 0, 0, 0};   // bricked floats per level and snapshot
     int nbx[kMaxLev] = {0, 0, 0, 0, 0, 0}, nby[kMaxLev] = {0, 0, 0, 0, 0, 0};
     std::vector<float> faces_packed, centres_packed;
-    int foff[kMaxLev][3], coff[3];
+    std::vector<int> offs_packed;   // brick-layout offset of each cell index, per level and axis
+    int foff[kMaxLev][3], coff[3], ooff[kMaxLev][3];
+    DevBuf<int> d_offs;
     DevBuf<float> d_faces, d_centres;
     bool geom_uploaded = false;
     // spline
@@ -237,6 +239,19 @@ static int upload_geometry(dnsmc_solver* s)
         s->tstride[l] = (long long)s->nbx[l] * s->nby[l] * ((n[2] + 3) / 4) * 64;
         HIPCHK(s->d_T[l].ensure((size_t)s->tstride[l] * s->cfg.nsnap));
     }
+    // brick_index(x, y, z) = ox[x] + oy[y] + oz[z] (disjoint bit fields), tabulated
+    s->offs_packed.clear();
+    for (size_t l = 0; l < s->lev.size(); ++l) {
+        const int* n = s->lev[l].n;
+        for (int a = 0; a < 3; ++a) {
+            s->ooff[l][a] = (int)s->offs_packed.size();
+            for (int c = 0; c < n[a]; ++c) {
+                const int x = a == 0 ? c : 0, y = a == 1 ? c : 0, z = a == 2 ? c : 0;
+                s->offs_packed.push_back(brick_index(x, y, z, s->nbx[l], s->nby[l]));
+            }
+        }
+    }
+    HIPCHK(s->d_offs.upload(s->offs_packed, s->stream));
     s->geom_uploaded = true;
     return DNSMC_OK;
 }
@@ -568,6 +583,7 @@ static int do_run(dnsmc_solver* s)
         P.lev[l].tstride = s->tstride[l];
         P.lev[l].nbx = s->nbx[l];
         P.lev[l].nby = s->nby[l];
+        for (int a = 0; a < 3; ++a) P.lev[l].ooff[a] = s->ooff[l][a];
         P.T[l] = s->d_T[l].p;
     }
     P.bc_periodic = 0;
@@ -585,6 +601,8 @@ static int do_run(dnsmc_solver* s)
     P.Ibw = s->d_Ibw.p;
     P.faces = s->d_faces.p;
     P.nfaces = (int)s->faces_packed.size();
+    P.offs = s->d_offs.p;
+    P.noffs = (int)s->offs_packed.size();
     P.centres = s->d_centres.p;
     for (int a = 0; a < 3; ++a) P.coff[a] = s->coff[a];
     P.ncent = (int)s->centres_packed.size();
@@ -595,7 +613,7 @@ static int do_run(dnsmc_solver* s)
     int tab = kTabGlobal;
     P.tab_lo = 0;
     P.tab_nw = d.nT;
-    size_t lds_trace = sizeof(float) * (6 * kLevWords + P.nfaces + 6 * (size_t)d.nB);
+    size_t lds_trace = sizeof(float) * (6 * kLevWords + P.nfaces + P.noffs + P.ncent + 6 * (size_t)d.nB);
     if (spec == kSpecTable) {
         float tlo = s->Tlo_snap[0], thi = s->Thi_snap[0];
         for (int i = 1; i < c.nsnap; ++i) {
@@ -664,6 +682,7 @@ static int do_run(dnsmc_solver* s)
     G.rays_per_slot = R;
     G.slots_per_emitter = spe;
     G.seed = c.seed;
+
     if (c.mode == DNSMC_MODE_COMPAT) {
         xorwow_seed_state(c.seed, G.xw_seed);
         G.xw_jump_sub = s->d_xw_sub.p;
@@ -693,6 +712,7 @@ static int do_run(dnsmc_solver* s)
             G.nslots = nsl;
             G.snap = snap;
             G.w0 = pp.d_w0.p; G.w1 = pp.d_w1.p; G.w = pp.d_w.p; G.winv = pp.d_winv.p;
+
             EventPair& eg = next_event(s, 0);
             HIPCHK(hipEventRecord(eg.a, st));
             if (c.mode == DNSMC_MODE_COMPAT) {
@@ -706,6 +726,7 @@ static int do_run(dnsmc_solver* s)
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(eg.b, st));
             P.w0 = pp.d_w0.p; P.w1 = pp.d_w1.p; P.w = pp.d_w.p; P.winv = pp.d_winv.p;
+
             P.pray = pp.d_pray.p;
             P.nrays = nr;
             P.slot0 = b0;
