@@ -64,7 +64,6 @@ struct TraceArgs {
     const float* w;
     const float* winv;
     long long nrays;
-    int chunk;
     int rays_per_slot;
     int slots_per_emitter;
     FastDiv fd_R;                         // division by rays per slot

@@ -100,7 +100,7 @@ typedef struct dnsmc_config {
     int nsnap;               /* temperature snapshots batched in one run */
     int precision;           /* 0 FP32 traversal, 1 FP64 traversal */
     int tally;               /* DNSMC_TALLY_* */
-    int chunk;               /* rays per lane chunk in the traversal */
+    int chunk;               /* reserved (accepted, unused) */
     long long batch_rays;    /* rays generated per traversal launch */
     char table_dir[512];     /* directory of the spectral tables */
 } dnsmc_config;
@@ -124,7 +124,8 @@ typedef struct dnsmc_stats {
 
 /* Defaults equal to the reference's shipped code/param.h + NarrowBand.h. */
 void dnsmc_default_config(dnsmc_config *cfg);
-/* Parse "key = value" lines (see dnsmc_amd/config.py for the keys). */
+/* Parse "key = value" lines; keys are the field names of dnsmc_config, lists
+ * space-separated ("n = 384 256 256", "bc = 1 1 2 2 1 1"), '#' starts a comment. */
 int dnsmc_parse_config(const char *text, dnsmc_config *cfg);
 
 int dnsmc_create(const dnsmc_config *cfg, dnsmc_solver **out);
