@@ -23,6 +23,8 @@ EXPORTED = [
     "dnsmc_set_emitters_slab", "dnsmc_set_emitters_list", "dnsmc_run", "dnsmc_wait",
     "dnsmc_get_results", "dnsmc_get_results_fortran", "dnsmc_get_stats", "dnsmc_get_array",
     "dnsmc_get_rays", "dnsmc_get_ray_records", "dnsmc_set_record", "dnsmc_sync", "dnsmc_bench_copy",
+    "dnsmc_mlp_nparams", "dnsmc_mlp_samples", "dnsmc_mlp_features", "dnsmc_mlp_grad", "dnsmc_mlp_adam",
+    "dnsmc_mlp_forward", "dnsmc_dev_alloc", "dnsmc_dev_free", "dnsmc_dev_copy",
 ]
 
 
@@ -110,6 +112,17 @@ def lib():
     L.dnsmc_set_record.argtypes = [vp, i32]
     L.dnsmc_sync.argtypes = []
     L.dnsmc_bench_copy.argtypes = [C.c_longlong, i32, f64p]
+    ll, u16p = C.c_longlong, C.c_void_p
+    L.dnsmc_mlp_nparams.argtypes = [i32]
+    L.dnsmc_mlp_nparams.restype = ll
+    L.dnsmc_mlp_samples.argtypes = [vp, vp, vp, vp, vp, ll, C.c_float]
+    L.dnsmc_mlp_features.argtypes = [vp, vp, ll]
+    L.dnsmc_mlp_grad.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, ll, vp, ll, vp, f32p]
+    L.dnsmc_mlp_adam.argtypes = [i32, vp, vp, vp, vp, C.c_float, i32, vp, vp, vp]
+    L.dnsmc_mlp_forward.argtypes = [i32, vp, vp, vp, vp, vp, ll, vp]
+    L.dnsmc_dev_alloc.argtypes = [ll, C.POINTER(vp)]
+    L.dnsmc_dev_free.argtypes = [vp]
+    L.dnsmc_dev_copy.argtypes = [vp, vp, ll, i32]
     L.mc_gpu_.argtypes = [f32p, C.POINTER(C.c_int)]
     L.mc_gpu_.restype = None
     L.get_results_.argtypes = [f32p, f32p]

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libdnsmc_rad.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["kernels.hip", "solver.cpp", "host.cpp"]
+SOURCES = ["kernels.hip", "mlp.hip", "solver.cpp", "host.cpp"]
 # -ffp-contract=off: only explicit fmaf() fuses (the solver's FP contract).
 # packed-fp32-ops off: hipcc must not emit v_pk_*_f32 from scalar float code.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

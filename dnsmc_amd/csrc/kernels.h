@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "dmath.h"
+#include "mlp.h"
 
 namespace dn {
 
@@ -127,7 +128,10 @@ struct GenArgs {
     int n0, n1, n2;                // level-0 sizes (reference stream decomposition)
     int num_streams, num_gpu, gpu, p_row, ystart;
     // learned proposal
-    const void* mlp;
+    MlpWeights mlpw;
+    FeatArgs feat;
+    const float* p_bg;             // plain probability of each (band, g) bin
+    float mix_eps;                 // defensive mixture weight of the plain distribution
     // outputs
     uint32_t* w0;
     uint32_t* w1;

@@ -35,6 +35,20 @@ __global__ void spline_pass(SplineArgs s);
 __global__ void reduce_rays(const float2* pray, int tally, long long nslots, int R, long long slot0,
                             const int4* slotinfo, double* E, double* A, double* Q2);
 __global__ void fill_f64(double* p, long long n, double v);
+// mlp.hip
+__global__ void mlp_proposal(GenArgs a, float* cdf);
+__global__ void raygen_learned(GenArgs a, const float* U, const float* pcdf);
+__global__ void mlp_train_tile(MlpTrainArgs a);
+__global__ void mlp_reduce(const float* part, const float* loss_part, int nblk, int P, float* grad, float* loss);
+__global__ void mlp_wsum(const float* wt, long long n, float* out);
+__global__ void mlp_adam(float* theta, float* m, float* v, const float* grad, int P, MlpLayout L, float lr, float b1,
+                         float b2, float eps, float c1, float c2, uint16_t* W1b, uint16_t* W2b, uint16_t* W3b);
+__global__ void mlp_forward_rows(const float* X, long long n, MlpWeights w, float* Z);
+__global__ void mlp_samples(FeatArgs f, const int4* slotinfo, long long slot0, int R, long long nrays, int nQ,
+                            const uint32_t* w0, const uint32_t* w1, const float* wv, const float* winv,
+                            const float* recD, const float* recG, const int* recS, float cost0, float* X, int* td,
+                            int* tb, float* wt);
+__global__ void mlp_features(FeatArgs f, const int4* slotinfo, long long n, float* X);
 __global__ void fill_u32(unsigned* p, long long n, unsigned v);
 __global__ void copy_f32x4(const float4* src, float4* dst, long long n4);
 }  // namespace dn
@@ -141,6 +155,7 @@ struct dnsmc_solver {
         DevBuf<uint32_t> d_w0, d_w1;
         DevBuf<float> d_w, d_winv;
         DevBuf<float2> d_pray;   // per-ray (E, A) for the deterministic tallies
+        DevBuf<float> d_qcdf;    // learned proposal: per-slot CDF rows of the batch
         DevBuf<unsigned> d_claim;   // per-XCD super-chunk counters of the traversal
         hipEvent_t done = nullptr;
     };
@@ -150,6 +165,12 @@ struct dnsmc_solver {
     // tallies and results
     DevBuf<double> d_E, d_A, d_Q2, d_Q, d_Eo, d_Ao, d_sd;
     DevBuf<unsigned long long> d_nsteps;
+    // learned proposal (BF16 weights, FP32 biases), plain bin probabilities
+    bool have_mlp = false;
+    int mlp_nbin = 0;
+    float mix_eps = 0.1f;
+    DevBuf<uint16_t> d_W1b, d_W2b, d_W3b;
+    DevBuf<float> d_b1, d_b2, d_b3, d_pbg, d_pcdf;
     // records
     int record = 0;
     long long nrec = 0;
@@ -180,7 +201,6 @@ static int check_config(const dnsmc_config& c)
                 return fail(DNSMC_EINVAL, "bc must be 1 (periodic) or 2 (black wall)");
     if (c.polar_axis < 0 || c.polar_axis > 2) return fail(DNSMC_EINVAL, "polar_axis must be 0, 1 or 2");
     if (c.mode < 0 || c.mode > 2) return fail(DNSMC_EINVAL, "unknown mode");
-    if (c.mode == DNSMC_MODE_LEARNED) return fail(DNSMC_EINVAL, "learned mode needs dnsmc_set_mlp first");
     if (c.spectral < 0 || c.spectral > 3) return fail(DNSMC_EINVAL, "unknown spectral model");
     if (c.mode == DNSMC_MODE_COMPAT && (c.nvar < 2 || c.photmax < 1))
         return fail(DNSMC_EINVAL, "compat mode needs nvar >= 2 and photmax >= 1");
@@ -364,6 +384,29 @@ static int ensure_derived(dnsmc_solver* s, float Tmax)
     HIPCHK(s->d_cdfb.upload(d.cdf_band, s->stream));
     HIPCHK(s->d_cdfg.upload(d.cdf_g, s->stream));
     HIPCHK(s->d_inv_emax.upload(d.inv_emax, s->stream));
+    {   // plain probability of each (band, g) bin: the CDF steps the plain draw uses
+        std::vector<float> pbg((size_t)d.nB * d.nQ, 0.f);
+        double pb0 = 0.0;
+        for (int b = 0; b < d.nB; ++b) {
+            const double pb = (double)d.cdf_band[b] - pb0;
+            pb0 = d.cdf_band[b];
+            double pg0 = 0.0;
+            for (int g = 0; g < d.nQ; ++g) {
+                const double pg = d.g_search ? (double)d.cdf_g[(size_t)b * d.nQ + g] - pg0 : (g == 0 ? 1.0 : 0.0);
+                if (d.g_search) pg0 = d.cdf_g[(size_t)b * d.nQ + g];
+                pbg[(size_t)b * d.nQ + g] = (float)(pb * pg);
+            }
+        }
+        HIPCHK(s->d_pbg.upload(pbg, s->stream));
+        std::vector<float> pcdf(pbg.size());   // inclusive CDF of pbg, last entry exactly 1
+        double acc = 0.0;
+        for (size_t i = 0; i < pbg.size(); ++i) {
+            acc += pbg[i];
+            pcdf[i] = (float)std::min(acc, 1.0);
+        }
+        if (!pcdf.empty()) pcdf.back() = 1.0f;
+        HIPCHK(s->d_pcdf.upload(pcdf, s->stream));
+    }
     s->der_Tmax = Tmax;
     return DNSMC_OK;
 }
@@ -496,6 +539,41 @@ static hipError_t launch_trace(int spec, int tab, bool det, bool rec, bool amc, 
     return launch_trace_m<kSpecTable, kTabGlobal>(det, rec, amc, ncu, max_blocks, lds, st, P);
 }
 
+static MlpWeights mlp_weights(dnsmc_solver* s)
+{
+    MlpWeights w;
+    w.W1 = s->d_W1b.p; w.b1 = s->d_b1.p;
+    w.W2 = s->d_W2b.p; w.b2 = s->d_b2.p;
+    w.W3 = s->d_W3b.p; w.b3 = s->d_b3.p;
+    w.nbin = s->mlp_nbin;
+    w.noutp = mlp_noutp(s->mlp_nbin);
+    return w;
+}
+
+// Feature arguments of the learned proposal (the field pointer is set per snapshot).
+static FeatArgs feat_args(dnsmc_solver* s, const Derived& d)
+{
+    const dnsmc_config& c = s->cfg;
+    FeatArgs f;
+    memset(&f, 0, sizeof f);
+    f.T0 = s->d_T[0].p;
+    for (int a = 0; a < 3; ++a) {
+        f.n[a] = c.n[a];
+        f.coff[a] = s->coff[a];
+    }
+    f.nbx = s->nbx[0];
+    f.nby = s->nby[0];
+    f.centres = s->d_centres.p;
+    for (int a = 0; a < 3; ++a)
+        for (int sd = 0; sd < 2; ++sd)
+            if (c.bc[a][sd] == DNSMC_BC_PERIODIC) f.bc_periodic |= 1 << (a * 2 + sd);
+    f.polar = c.polar_axis;
+    f.Lpolar = c.L[c.polar_axis];
+    f.inv_Tmax = 1.0f / d.Tmax;
+    f.Tw_rel = 0.5f * (c.Tw[c.polar_axis][0] + c.Tw[c.polar_axis][1]) / d.Tmax;
+    return f;
+}
+
 static int do_run(dnsmc_solver* s)
 {
     const dnsmc_config& c = s->cfg;
@@ -528,10 +606,15 @@ static int do_run(dnsmc_solver* s)
     // ray batches: up to 2^30 rays (16 GB of descriptors) per launch; the
     // deterministic tallies keep 8 more bytes per ray, so they use 2^27
     const long long cap = det ? (1ll << 27) : (1ll << 30);
-    const long long slots_per_batch = std::max<long long>(1, std::min<long long>(c.batch_rays, cap) / R);
+    long long slots_per_batch = std::max<long long>(1, std::min<long long>(c.batch_rays, cap) / R);
+    if (c.mode == DNSMC_MODE_LEARNED && s->have_mlp)   // CDF rows: at most 8 GB per pipeline
+        slots_per_batch = std::max<long long>(1, std::min<long long>(slots_per_batch,
+                                                                     (8ll << 30) / (4ll * mlp_noutp(s->mlp_nbin))));
     const long long batch_rays = std::min(slots_per_batch, nslots) * R;
     const long long nbatch_total = ((nslots + slots_per_batch - 1) / slots_per_batch) * c.nsnap;
-    const int npipes = (int)std::min<long long>(dnsmc_solver::kPipes, nbatch_total);
+    int pipes_max = dnsmc_solver::kPipes;   // DNSMC_PIPES=1 serialises the batches (standalone kernel timing)
+    if (const char* pe = getenv("DNSMC_PIPES")) pipes_max = std::max(1, std::min(dnsmc_solver::kPipes, atoi(pe)));
+    const int npipes = (int)std::min<long long>(pipes_max, nbatch_total);
     for (int p = 0; p < npipes; ++p) {
         dnsmc_solver::Pipe& pp = s->pipe[p];
         HIPCHK(pp.d_w0.ensure(batch_rays));
@@ -539,6 +622,8 @@ static int do_run(dnsmc_solver* s)
         HIPCHK(pp.d_w.ensure(batch_rays));
         HIPCHK(pp.d_winv.ensure(batch_rays));
         if (det) HIPCHK(pp.d_pray.ensure(batch_rays));
+        if (c.mode == DNSMC_MODE_LEARNED && s->have_mlp)
+            HIPCHK(pp.d_qcdf.ensure((size_t)std::min(slots_per_batch, nslots) * mlp_noutp(s->mlp_nbin)));
         HIPCHK(pp.d_claim.ensure(256));
     }
     HIPCHK(s->d_E.ensure(nslots * c.nsnap));
@@ -697,6 +782,17 @@ static int do_run(dnsmc_solver* s)
     }
     const bool gib = spec == kSpecTable && (size_t)d.nB * d.nT <= 16384;
     const size_t lds_gen = sizeof(float) * ((size_t)d.nB + (size_t)d.nB * d.nQ + (gib ? (size_t)d.nB * d.nT : 0) + 64);
+    size_t lds_learn = 0;
+    if (c.mode == DNSMC_MODE_LEARNED) {
+        if (!s->have_mlp) return fail(DNSMC_ESTATE, "learned mode needs dnsmc_set_mlp first");
+        if (s->mlp_nbin != d.nB * d.nQ) return fail(DNSMC_EINVAL, "proposal network does not match the tables' bins");
+        G.mlpw = mlp_weights(s);
+        G.feat = feat_args(s, d);
+        G.p_bg = s->d_pbg.p;
+        G.mix_eps = s->mix_eps;
+        lds_learn = sizeof(float) * (8 * (size_t)G.mlpw.noutp + G.mlpw.nbin);
+        if (lds_learn > s->lds_max) return fail(DNSMC_EINVAL, "too many (band, g) bins for the learned proposal");
+    }
 
     const int trace_blocks = s->ncu * 16;
     s->first_batch_rays = 0;
@@ -715,7 +811,14 @@ static int do_run(dnsmc_solver* s)
 
             EventPair& eg = next_event(s, 0);
             HIPCHK(hipEventRecord(eg.a, st));
-            if (c.mode == DNSMC_MODE_COMPAT) {
+            if (c.mode == DNSMC_MODE_LEARNED) {
+                const size_t lds_prop = sizeof(float) * (64 * kMlpIn + 4096 + 64) + 4 * 4096 * sizeof(uint16_t);
+                int nb = (int)std::min<long long>((nsl + 63) / 64, (long long)s->ncu * 3);
+                hipLaunchKernelGGL(mlp_proposal, dim3(nb), dim3(256), lds_prop, st, G, pp.d_qcdf.p);
+                nb = (int)std::min<long long>((nsl + 7) / 8, (long long)s->ncu * 3);
+                hipLaunchKernelGGL(raygen_learned, dim3(nb), dim3(256), lds_learn, st, G, (const float*)pp.d_qcdf.p,
+                                   (const float*)s->d_pcdf.p);
+            } else if (c.mode == DNSMC_MODE_COMPAT) {
                 long long nth = nsl * ((R + G.sub_len - 1) / G.sub_len);
                 int nb = (int)std::min<long long>((nth + 255) / 256, 1 << 20);
                 hipLaunchKernelGGL(raygen_compat, dim3(nb), dim3(256), lds_gen, st, G);
@@ -1007,10 +1110,35 @@ int dnsmc_set_field(dnsmc_solver* s, int snap, const float* T, int is_device)
     return DNSMC_OK;
 }
 
-int dnsmc_set_mlp(dnsmc_solver*, const unsigned short*, const float*, const unsigned short*, const float*,
-                  const unsigned short*, const float*, float)
+int dnsmc_set_mlp(dnsmc_solver* s, const unsigned short* W1, const float* b1, const unsigned short* W2, const float* b2,
+                  const unsigned short* W3, const float* b3, float mix_eps)
 {
-    return fail(DNSMC_EINVAL, "learned proposal not available in this build");
+    if (!s || !W1 || !b1 || !W2 || !b2 || !W3 || !b3) return fail(DNSMC_EINVAL, "null argument");
+    if (!s->have_tables) return fail(DNSMC_ESTATE, "load the spectral tables before the proposal network");
+    if (!(mix_eps > 0.f && mix_eps <= 1.f)) return fail(DNSMC_EINVAL, "mix_eps must be in (0, 1]");
+    if (s->running) {
+        int rc = do_wait(s);
+        if (rc) return rc;
+    }
+    const int nbin = s->cfg.spectral == DNSMC_SPEC_GREY_T4 ? 1 : s->tab.nB * s->tab.nQ;
+    const int noutp = mlp_noutp(nbin);
+    auto up16 = [&](DevBuf<uint16_t>& d, const unsigned short* h, size_t n) {
+        return d.upload(std::vector<uint16_t>(h, h + n), s->stream);
+    };
+    auto up32 = [&](DevBuf<float>& d, const float* h, size_t n) {
+        return d.upload(std::vector<float>(h, h + n), s->stream);
+    };
+    HIPCHK(up16(s->d_W1b, W1, (size_t)kMlpIn * kMlpHid));
+    HIPCHK(up32(s->d_b1, b1, kMlpHid));
+    HIPCHK(up16(s->d_W2b, W2, (size_t)kMlpHid * kMlpHid));
+    HIPCHK(up32(s->d_b2, b2, kMlpHid));
+    HIPCHK(up16(s->d_W3b, W3, (size_t)kMlpHid * noutp));
+    HIPCHK(up32(s->d_b3, b3, noutp));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->have_mlp = true;
+    s->mlp_nbin = nbin;
+    s->mix_eps = mix_eps;
+    return DNSMC_OK;
 }
 
 int dnsmc_set_emitters_slab(dnsmc_solver* s, int ystart)
@@ -1225,6 +1353,152 @@ int dnsmc_bench_copy(long long nbytes, int reps, double* gbps)
 
 int dnsmc_sync(void)
 {
+    HIPCHK(hipDeviceSynchronize());
+    return DNSMC_OK;
+}
+
+// ------------------------------------------------------ device buffers ----
+// Plain device allocations for callers that hold training data on the GPU
+// without another runtime (bindings, tests).
+int dnsmc_dev_alloc(long long bytes, void** out)
+{
+    if (!out || bytes < 0) return fail(DNSMC_EINVAL, "bad arguments");
+    *out = nullptr;
+    HIPCHK(hipMalloc(out, (size_t)std::max<long long>(bytes, 1)));
+    return DNSMC_OK;
+}
+
+int dnsmc_dev_free(void* p)
+{
+    if (p) HIPCHK(hipFree(p));
+    return DNSMC_OK;
+}
+
+// kind: 1 host -> device, 2 device -> host, 3 device -> device (synchronous)
+int dnsmc_dev_copy(void* dst, const void* src, long long bytes, int kind)
+{
+    if (!dst || !src || bytes < 0 || kind < 1 || kind > 3) return fail(DNSMC_EINVAL, "bad arguments");
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    HIPCHK(hipMemcpy(dst, src, (size_t)bytes, k));
+    return DNSMC_OK;
+}
+
+// ------------------------------------------- learned proposal training ----
+// All pointers below are device pointers; the calls run on the null stream
+// and return when their results are ready.
+
+long long dnsmc_mlp_nparams(int nbin) { return nbin < 1 ? -1 : mlp_layout(nbin).P; }
+
+int dnsmc_mlp_samples(dnsmc_solver* s, float* X, int* td, int* tb, float* wt, long long n, float cost0)
+{
+    if (!s || !X || !td || !tb || !wt) return fail(DNSMC_EINVAL, "null argument");
+    int rc = do_wait(s);
+    if (rc) return rc;
+    const dnsmc_config& c = s->cfg;
+    if (!s->record) return fail(DNSMC_ESTATE, "samples need a run with records on (dnsmc_set_record)");
+    const long long nr = s->first_batch_rays;
+    if (nr <= 0 || nr > s->nrec || c.nsnap != 1 ||
+        (long long)s->emit.size() * spe_of(c) * rays_per_slot_of(c) != nr)
+        return fail(DNSMC_ESTATE, "samples need the last run to be one snapshot in one batch of <= 2^22 rays");
+    if (n < nr) return fail(DNSMC_EINVAL, "sample arrays must hold every ray of the run");
+    const dnsmc_solver::Pipe& p = s->pipe[0];
+    FeatArgs f = feat_args(s, s->der);
+    int nb = (int)std::min<long long>((nr + 255) / 256, (long long)s->ncu * 64);
+    hipLaunchKernelGGL(mlp_samples, dim3(nb), dim3(256), 0, 0, f, (const int4*)s->d_slotinfo.p, 0ll,
+                       rays_per_slot_of(c), nr, s->der.nQ, (const uint32_t*)p.d_w0.p, (const uint32_t*)p.d_w1.p,
+                       (const float*)p.d_w.p, (const float*)p.d_winv.p, (const float*)s->d_rD.p,
+                       (const float*)s->d_rG.p, (const int*)s->d_rsteps.p, cost0, X, td, tb, wt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    return (int)std::min<long long>(nr, 0x7FFFFFFF);
+}
+
+int dnsmc_mlp_features(dnsmc_solver* s, float* X, long long n)
+{
+    if (!s || !X) return fail(DNSMC_EINVAL, "null argument");
+    int rc = do_wait(s);
+    if (rc) return rc;
+    if (s->der_Tmax <= 0.f || s->emit_dirty) return fail(DNSMC_ESTATE, "run the solver first");
+    const long long ns = (long long)s->order.size() * spe_of(s->cfg);
+    n = std::min(n, ns);
+    FeatArgs f = feat_args(s, s->der);
+    int nb = (int)std::min<long long>((n + 255) / 256, (long long)s->ncu * 64);
+    hipLaunchKernelGGL(mlp_features, dim3(nb), dim3(256), 0, 0, f, (const int4*)s->d_slotinfo.p, n, X);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    return (int)n;
+}
+
+static MlpWeights mlp_weights_of(int nbin, const unsigned short* W1b, const unsigned short* W2b,
+                                 const unsigned short* W3b, const float* theta)
+{
+    const MlpLayout L = mlp_layout(nbin);
+    MlpWeights w;
+    w.W1 = W1b; w.W2 = W2b; w.W3 = W3b;
+    w.b1 = theta + L.ob1; w.b2 = theta + L.ob2; w.b3 = theta + L.ob3;
+    w.nbin = nbin;
+    w.noutp = L.noutp;
+    return w;
+}
+
+// Loss and gradient of the weighted cross-entropy over n samples (weights as
+// BF16 copies, biases inside theta).  work: at least
+// ceil(n/64) * (P + 1) + P + 2 floats.
+int dnsmc_mlp_grad(int nbin, const float* theta, const unsigned short* W1b, const unsigned short* W2b,
+                   const unsigned short* W3b, const float* X, const int* td, const int* tb, const float* wt, long long n,
+                   float* work, long long nwork, float* grad, float* loss)
+{
+    if (nbin < 1 || !theta || !W1b || !W2b || !W3b || !X || !td || !tb || !wt || !work || !grad || n < 1)
+        return fail(DNSMC_EINVAL, "bad arguments");
+    const MlpLayout L = mlp_layout(nbin);
+    const long long nblk = (n + 63) / 64;
+    if (nblk > 0x7FFFFFFF || nwork < nblk * (L.P + 1) + L.P + 2) return fail(DNSMC_EINVAL, "work buffer too small");
+    float* part = work;
+    float* lpart = part + nblk * L.P;
+    float* wsum = lpart + nblk;
+    float* dloss = wsum + 1;
+    MlpTrainArgs a;
+    a.X = X; a.td = td; a.tb = tb; a.wt = wt; a.n = (int)std::min<long long>(n, 0x7FFFFFFF);
+    a.wsum = wsum;
+    a.w = mlp_weights_of(nbin, W1b, W2b, W3b, theta);
+    a.part = part;
+    a.loss_part = lpart;
+    hipLaunchKernelGGL(mlp_wsum, dim3(1), dim3(256), 0, 0, wt, n, wsum);
+    const size_t lds = sizeof(float) * (64 * kMlpIn + 3 * 4096 + 8 * 64 + 2 * 64) + 5 * 4096 * sizeof(uint16_t);
+    hipLaunchKernelGGL(mlp_train_tile, dim3((unsigned)nblk), dim3(256), lds, 0, a);
+    hipLaunchKernelGGL(mlp_reduce, dim3((L.P + 255) / 256), dim3(256), 0, 0, part, lpart, (int)nblk, L.P, grad, dloss);
+    HIPCHK(hipGetLastError());
+    if (loss) HIPCHK(hipMemcpy(loss, dloss, sizeof(float), hipMemcpyDeviceToHost));
+    else HIPCHK(hipDeviceSynchronize());
+    return DNSMC_OK;
+}
+
+// One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on theta; refreshes the
+// BF16 weight copies.  step counts from 1.
+int dnsmc_mlp_adam(int nbin, float* theta, float* m, float* v, const float* grad, float lr, int step,
+                   unsigned short* W1b, unsigned short* W2b, unsigned short* W3b)
+{
+    if (nbin < 1 || !theta || !m || !v || !grad || !W1b || !W2b || !W3b || step < 1)
+        return fail(DNSMC_EINVAL, "bad arguments");
+    const MlpLayout L = mlp_layout(nbin);
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = (float)(1.0 / (1.0 - pow((double)b1, step))), c2 = (float)(1.0 / (1.0 - pow((double)b2, step)));
+    hipLaunchKernelGGL(mlp_adam, dim3((L.P + 255) / 256), dim3(256), 0, 0, theta, m, v, grad, L.P, L, lr, b1, b2, 1e-8f,
+                       c1, c2, W1b, W2b, W3b);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    return DNSMC_OK;
+}
+
+// Logits Z[n][noutp] of the network for inputs X[n][6].
+int dnsmc_mlp_forward(int nbin, const float* theta, const unsigned short* W1b, const unsigned short* W2b,
+                      const unsigned short* W3b, const float* X, long long n, float* Z)
+{
+    if (nbin < 1 || !theta || !W1b || !W2b || !W3b || !X || !Z || n < 1) return fail(DNSMC_EINVAL, "bad arguments");
+    const MlpWeights w = mlp_weights_of(nbin, W1b, W2b, W3b, theta);
+    const size_t lds = sizeof(float) * 64 * kMlpIn + 4 * 4096 * sizeof(uint16_t);
+    hipLaunchKernelGGL(mlp_forward_rows, dim3((unsigned)((n + 63) / 64)), dim3(256), lds, 0, X, n, w, Z);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     return DNSMC_OK;
 }

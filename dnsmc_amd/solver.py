@@ -66,6 +66,14 @@ class Solver:
         self.n = tuple(cfg.n)
         self.nemit = self.n[0] * self.n[1] * self.n[2]
 
+    @property
+    def rays_per_slot(self):
+        return self.cfg.photmax if self.cfg.mode == _lib.MODE_COMPAT else self.cfg.rays_per_cell
+
+    @property
+    def nslots(self):
+        return self.nemit * (self.cfg.nvar if self.cfg.mode == _lib.MODE_COMPAT else 1)
+
     def close(self):
         if self.h:
             lib().dnsmc_destroy(self.h)
@@ -105,6 +113,13 @@ class Solver:
         cells = np.ascontiguousarray(cells, np.int32)
         check(lib().dnsmc_set_emitters_list(self.h, cells.ctypes.data_as(C.POINTER(C.c_int)), len(cells)))
         self.nemit = len(cells)
+
+    def set_mlp(self, W1, b1, W2, b2, W3, b3, mix_eps=0.1):
+        """Learned proposal weights: W* as BF16 bit patterns (uint16, row-major
+        [in][out]), b* float32; see include/dnsmc_rad.h:dnsmc_set_mlp."""
+        a = [np.ascontiguousarray(x, np.uint16 if i % 2 == 0 else np.float32)
+             for i, x in enumerate((W1, b1, W2, b2, W3, b3))]
+        check(lib().dnsmc_set_mlp(self.h, *[x.ctypes.data for x in a], C.c_float(mix_eps)))
 
     def set_record(self, on=True):
         check(lib().dnsmc_set_record(self.h, 1 if on else 0))

@@ -145,7 +145,12 @@ int dnsmc_set_tables(dnsmc_solver *s, int nB, int nQ, int nT, const float *Tnb, 
 /* Temperature field of snapshot `snap`, Fortran layout with ghosts
  * (n0+2)(n1+2)(n2+2), host or device pointer (is_device). */
 int dnsmc_set_field(dnsmc_solver *s, int snap, const float *T, int is_device);
-/* Learned proposal weights (BF16 bits, row-major [in][out]) + FP32 biases. */
+/* Learned proposal (no counterpart in the reference): network 6 -> 64 -> 64 ->
+ * (64 direction bins | nbin = nB*nQ (band, g) bins, padded to noutp, a multiple
+ * of 64).  Weights as BF16 bits, row-major [in][out] (W1 6x64, W2 64x64,
+ * W3 64 x noutp), biases FP32 (64, 64, noutp); host pointers; the tables must
+ * be loaded.  mix_eps in (0, 1]: the rays are drawn from
+ * (1 - mix_eps) q_net + mix_eps p_plain and reweighted by p/q. */
 int dnsmc_set_mlp(dnsmc_solver *s, const unsigned short *W1, const float *b1, const unsigned short *W2,
                   const float *b2, const unsigned short *W3, const float *b3, float mix_eps);
 
@@ -177,6 +182,38 @@ long long dnsmc_get_ray_records(dnsmc_solver *s, int *steps, int *end_type, unsi
                                 float *D, float *G, long long n);
 /* Switch on per-ray records (costs bandwidth; off by default). */
 int dnsmc_set_record(dnsmc_solver *s, int on);
+/* Training hooks of the learned proposal (no counterpart in the reference; used
+ * by dnsmc_amd/learned.py).  Pointers are device pointers.  Parameters are one
+ * FP32 vector theta of dnsmc_mlp_nparams(nbin) floats: W1, b1, W2, b2, W3, b3
+ * in that order with the shapes above; W1b/W2b/W3b are its BF16 copies. */
+long long dnsmc_mlp_nparams(int nbin);
+/* Samples of the last run (one snapshot, one batch of <= 2^22 rays, records
+ * on): per ray its cell's 6 features, direction bin, (band, g) bin and weight
+ * |x| / sqrt(steps + cost0) with x the ray's net contribution (|x| when
+ * cost0 < 0): the fit then targets the proposal of least variance x cost. */
+int dnsmc_mlp_samples(dnsmc_solver *s, float *X, int *dir_bin, int *bin, float *weight, long long n,
+                      float cost0);
+/* Features of the first n emitting slots (processing order) of the last run. */
+int dnsmc_mlp_features(dnsmc_solver *s, float *X, long long n);
+/* Loss (sum w CE / sum w over both heads) and its gradient over n samples;
+ * work holds ceil(n/64)*(P+1) + P + 2 floats; loss is a host pointer (may be NULL). */
+int dnsmc_mlp_grad(int nbin, const float *theta, const unsigned short *W1b, const unsigned short *W2b,
+                   const unsigned short *W3b, const float *X, const int *dir_bin, const int *bin,
+                   const float *weight, long long n, float *work, long long nwork, float *grad, float *loss);
+/* One Adam step (0.9, 0.999, 1e-8) on theta, refreshing the BF16 copies. */
+int dnsmc_mlp_adam(int nbin, float *theta, float *m, float *v, const float *grad, float lr, int step,
+                   unsigned short *W1b, unsigned short *W2b, unsigned short *W3b);
+/* Logits Z[n][noutp] for inputs X[n][6]. */
+int dnsmc_mlp_forward(int nbin, const float *theta, const unsigned short *W1b, const unsigned short *W2b,
+                      const unsigned short *W3b, const float *X, long long n, float *Z);
+
+/* Device buffers for bindings that keep data on the GPU (training samples):
+ * allocate, free, synchronous copy (kind 1 host->device, 2 device->host,
+ * 3 device->device). */
+int dnsmc_dev_alloc(long long bytes, void **out);
+int dnsmc_dev_free(void *p);
+int dnsmc_dev_copy(void *dst, const void *src, long long bytes, int kind);
+
 /* Device synchronise (bench timing). */
 int dnsmc_sync(void);
 /* Measurement hook: `reps` launches of a plain device-to-device copy of

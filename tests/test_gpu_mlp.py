@@ -1,0 +1,186 @@
+"""GPU tests of the learned proposal: the BF16 MFMA forward and backward
+kernels against the FP64 oracle (oracle/mlp.c, itself pinned by finite
+differences), training on plain-run tallies, unbiasedness of the p/q
+reweighted estimator and its variance against the plain estimator.
+
+Tolerances: BF16 weights are exact inputs to both sides; the kernels round the
+activations H1, H2 and dZ to BF16 (8 mantissa bits), so logits agree to
+~1e-2 of their scale and gradients to a few 1e-2 in relative norm.
+"""
+import numpy as np
+import pytest
+
+import dnsmc_amd
+from dnsmc_amd import learned, workloads
+from dnsmc_amd.devarray import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+
+def t(a):
+    return DeviceArray.from_numpy(np.ascontiguousarray(a))
+
+
+def _random_net(nbin, seed):
+    net = learned.ProposalNet(nbin, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    th = net.theta.numpy()
+    L = net.L
+    th[L["W3"]:L["b3"]] = rng.uniform(-1, 1, L["b3"] - L["W3"]) * 0.3     # non-trivial logits
+    th[L["b1"]:L["W2"]] = rng.normal(0, 0.1, 64)
+    th[L["b3"]:L["P"]] = rng.normal(0, 0.2, L["P"] - L["b3"])
+    return learned.ProposalNet(nbin, theta=th)
+
+
+def _oracle_params(net):
+    """The network's parameters as the kernels see them (BF16 weights) in FP64."""
+    th = net.theta.numpy().astype(np.float64)
+    L, no = net.L, net.L["noutp"]
+    W1 = learned.bf16_to_f32(net.W1b.numpy()).reshape(6, 64).astype(np.float64)
+    W2 = learned.bf16_to_f32(net.W2b.numpy()).reshape(64, 64).astype(np.float64)
+    W3 = learned.bf16_to_f32(net.W3b.numpy()).reshape(64, no).astype(np.float64)
+    nout = 64 + net.nbin
+    return [W1, th[L["b1"]:L["W2"]], W2, th[L["b2"]:L["W3"]], W3[:, :nout], th[L["b3"]:L["b3"] + nout]]
+
+
+@pytest.mark.parametrize("nbin", [119, 1600])
+def test_forward_matches_oracle(oracle, nbin):
+    net = _random_net(nbin, 3)
+    rng = np.random.default_rng(5)
+    X = rng.normal(0, 1, (300, 6)).astype(np.float32)
+    Z = net.forward(t(X)).numpy()
+    _, _, Zr = oracle.mlp_forward(X.astype(np.float64), *_oracle_params(net))
+    nout = 64 + nbin
+    err = np.abs(Z[:, :nout] - Zr)
+    assert np.max(err) <= 2e-2 * np.max(np.abs(Zr)) + 1e-3, np.max(err)
+
+
+@pytest.mark.parametrize("nbin", [119, 1600])
+def test_gradients_match_oracle(oracle, nbin):
+    net = _random_net(nbin, 7)
+    rng = np.random.default_rng(9)
+    n = 1000
+    X = rng.normal(0, 1, (n, 6)).astype(np.float32)
+    td = rng.integers(0, 64, n).astype(np.int32)
+    tb = rng.integers(0, nbin, n).astype(np.int32)
+    wt = rng.random(n).astype(np.float32)
+    wt[rng.random(n) < 0.3] = 0.0
+    loss = net.loss_grad(t(X), t(td), t(tb), t(wt))
+    g = net.grad.numpy()
+    p = _oracle_params(net)
+    lr, gr = oracle.mlp_loss_grad(X.astype(np.float64), *p, 64, td, tb, wt.astype(np.float64))
+    assert loss == pytest.approx(lr, rel=1e-2)
+    L, no = net.L, net.L["noutp"]
+    nout = 64 + nbin
+    blocks = [
+        ("W1", g[L["W1"]:L["b1"]], gr[0].ravel()),
+        ("b1", g[L["b1"]:L["W2"]], gr[1]),
+        ("W2", g[L["W2"]:L["b2"]], gr[2].ravel()),
+        ("b2", g[L["b2"]:L["W3"]], gr[3]),
+        ("W3", g[L["W3"]:L["b3"]].reshape(64, no)[:, :nout].ravel(), gr[4].ravel()),
+        ("b3", g[L["b3"]:L["b3"] + nout], gr[5]),
+    ]
+    for name, got, want in blocks:
+        rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+        assert rel < 3e-2, (name, rel)
+    # padding columns get no gradient
+    assert np.all(g[L["W3"]:L["b3"]].reshape(64, no)[:, nout:] == 0.0)
+
+
+def test_training_fits_a_known_distribution():
+    """Samples drawn from a feature-dependent categorical distribution: the
+    trained network's softmax approaches it (loss near the entropy)."""
+    nbin = 40
+    rng = np.random.default_rng(1)
+    n = 64 * 1024
+    X = rng.normal(0, 1, (n, 6)).astype(np.float32)
+    logit_d = np.outer(X[:, 0], np.linspace(-2, 2, 64))
+    logit_b = np.outer(X[:, 1], np.linspace(-1.5, 1.5, nbin))
+    def draw(lg):
+        p = np.exp(lg - lg.max(1, keepdims=True))
+        p /= p.sum(1, keepdims=True)
+        c = p.cumsum(1)
+        u = rng.random((lg.shape[0], 1))
+        return (u > c).sum(1).clip(0, lg.shape[1] - 1), p
+    td, pd = draw(logit_d)
+    tb, pb = draw(logit_b)
+    ent = np.mean(-np.sum(pd * np.log(pd + 1e-30), 1) - np.sum(pb * np.log(pb + 1e-30), 1))
+    samples = (t(X), t(td.astype(np.int32)), t(tb.astype(np.int32)), t(np.ones(n, np.float32)))
+    net = learned.ProposalNet(nbin, seed=2)
+    hist = learned.train(net, samples, epochs=6, batch=4096, lr=3e-3, seed=3)
+    assert hist[-1] < hist[0]
+    assert hist[-1] < ent + 0.05 * ent, (hist, ent)
+
+
+def _small_case(table_dir, mode, R, tally=0, n=(16, 24, 16)):
+    return workloads.channel_config(n, rays_per_cell=R, spectral=0, Tw=573.0, table_dir=table_dir,
+                                    mode=mode, tally=tally)
+
+
+def _field(seed, n):
+    return workloads.ghosted(workloads.field_fourier(seed, n))
+
+
+@pytest.fixture(scope="module")
+def trained(ck_table_dir):
+    """A network trained on plain tallies of three snapshots of a small channel."""
+    tdir, _ = ck_table_dir
+    n = (16, 24, 16)
+    samples = learned.SampleSet()
+    for seed in (31, 32, 33):
+        S = dnsmc_amd.Solver(_small_case(tdir, 0, 32, n=n))
+        S.load_tables()
+        S.set_record(True)
+        S.set_field(0, _field(seed, n))
+        S.run()
+        S.wait()
+        samples.add_run(S)
+        nbin = int(S.array("scalars")[3]) * int(S.array("scalars")[4])
+        S.close()
+    net = learned.ProposalNet(nbin, seed=0)
+    hist = learned.train(net, samples.cat(), epochs=4, batch=8192, lr=2e-3, seed=0)
+    return net, hist, tdir, n
+
+
+def _run(tdir, n, mode, R, field, net=None, seed=20240601):
+    cfg = _small_case(tdir, mode, R, n=n)
+    cfg.seed = seed
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    if net is not None:
+        S.set_mlp(*net.host_weights(), mix_eps=0.1)
+    S.set_field(0, field)
+    S.run()
+    out = S.results(0)
+    S.close()
+    return out
+
+
+def test_training_on_tallies_reduces_loss(trained):
+    _, hist, _, _ = trained
+    assert hist[-1] < hist[0]
+
+
+def test_learned_estimator_is_unbiased_and_reduces_variance(trained):
+    """Held-out snapshot, equal ray budget.  Per-cell z-scores are not used
+    as the bar: the plain estimator is heavy-tailed (rare long rays to the cold
+    walls), so its own per-cell variance estimate is occasionally far too small
+    at 256 rays.  Sums over cells are Gaussian: the bar is the z of the total
+    and of every wall-normal row (256 cells each), and the spread of the
+    per-cell z."""
+    net, _, tdir, n = trained
+    field = _field(51, n)                                    # held-out snapshot
+    plain = _run(tdir, n, 0, 256, field, seed=101)
+    lrn = _run(tdir, n, 2, 256, field, net=net, seed=202)
+    var_p, var_l = plain["sdev"] ** 2, lrn["sdev"] ** 2
+    dq = lrn["Q"] - plain["Q"]
+    z = dq / np.sqrt(var_p + var_l)
+    assert abs(np.mean(z)) < 0.1 and np.std(z) < 1.3, (np.mean(z), np.std(z))
+    ztot = np.sum(dq) / np.sqrt(np.sum(var_p + var_l))
+    assert abs(ztot) < 4.0, ztot
+    rows = dq.reshape(n[2], n[1], n[0]).sum(axis=(0, 2))
+    rvar = (var_p + var_l).reshape(n[2], n[1], n[0]).sum(axis=(0, 2))
+    zrow = rows / np.sqrt(rvar)
+    assert np.max(np.abs(zrow)) < 4.5, zrow
+    ratio = np.sum(var_p) / np.sum(var_l)
+    assert ratio > 3.0, ratio
