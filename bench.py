@@ -100,6 +100,26 @@ def cpu_baseline(S, cfg, fields, R, budget_s):
                        % (k2, R, dt, nsteps / dt))
 
 
+def measured_traffic(args, cfg, R):
+    """HBM bytes per traversal launch measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE, each scaled by its reading for a copy kernel of known size) on
+    this same workload; the summary in profiles/ records the command."""
+    path = os.path.join(ROOT, "profiles", "trace_%s_traffic.txt" % args.workload)
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    with open(path) as f:
+        for line in f:
+            if "=" in line and not line.startswith("#"):
+                k, v = line.split("=", 1)
+                kv[k.strip()] = v.strip()
+    want = "%s rays=%d nsnap=%d batch=%s" % (args.workload, R, cfg.nsnap, args.batch)
+    if kv.get("workload") != want or "traffic_bytes_per_launch" not in kv:
+        return None
+    return {"traffic_bytes_per_launch": float(kv["traffic_bytes_per_launch"]), "l2_hit_rate": float(kv["l2_hit_rate"]),
+            "source": os.path.relpath(path, ROOT)}
+
+
 def train_proposal(n_snap, rays, cost0, epochs):
     """The learned proposal of the channel: trained by the library's MFMA
     kernels on the plain-run tallies of n_snap seeded 64x96x64 snapshots
@@ -253,9 +273,21 @@ def main():
         "trace_launches_per_step": launches // args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "trace", "algorithmic_bytes_per_launch": b_alg / max(launches, 1)},
+                     "kernel": "trace", "algorithmic_bytes_per_launch": b_alg / max(launches, 1),
+                     "launch_ms_avg": ms_trace / max(launches, 1),
+                     "concurrent_launches": "up to %d (one per pipeline stream)" % st["n_pipes"],
+                     "achieved_whole_device": b_alg / dt / 1e9},
         "cpu_baseline": None,
     }
+    prof = measured_traffic(args, cfg, R)
+    if prof:
+        tb = prof["traffic_bytes_per_launch"]
+        out["roofline"]["traffic"] = tb
+        out["roofline"]["traffic_source"] = prof["source"]
+        out["roofline"]["l2_hit_rate"] = prof["l2_hit_rate"]
+        # measured HBM traffic (cache-line granular gathers, L2 misses) per second
+        out["roofline"]["traffic_GBs_per_launch"] = tb / (ms_trace / max(launches, 1) * 1e-3) / 1e9
+        out["roofline"]["traffic_GBs_whole_device"] = tb * launches / dt / 1e9
     if args.copy_gib > 0:
         import ctypes
         from dnsmc_amd._lib import check, lib
@@ -263,6 +295,8 @@ def main():
         check(lib().dnsmc_bench_copy(int(args.copy_gib * 2 ** 30), 5, ctypes.byref(g)))
         out["roofline"]["achievable_copy_GBs"] = g.value
         out["roofline"]["frac_of_achievable"] = achieved / g.value
+        if prof:
+            out["roofline"]["traffic_frac_of_achievable_whole_device"] = out["roofline"]["traffic_GBs_whole_device"] / g.value
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(S, cfg, fields, R, args.cpu_seconds)

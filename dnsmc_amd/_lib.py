@@ -70,7 +70,7 @@ class Stats(C.Structure):
         ("ms_total", C.c_double), ("ms_trace", C.c_double), ("ms_raygen", C.c_double), ("ms_other", C.c_double),
         ("n_rays", C.c_longlong), ("n_steps", C.c_longlong), ("n_trace_launches", C.c_int),
         ("Tmax", C.c_float), ("kappamax", C.c_float), ("Ibmax", C.c_double),
-        ("table_mode", C.c_int), ("table_lo", C.c_int), ("table_nw", C.c_int),
+        ("table_mode", C.c_int), ("table_lo", C.c_int), ("table_nw", C.c_int), ("n_pipes", C.c_int),
     ]
 
 

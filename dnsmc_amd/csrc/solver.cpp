@@ -875,6 +875,7 @@ static int do_run(dnsmc_solver* s)
     s->st.table_mode = tab;
     s->st.table_lo = P.tab_lo;
     s->st.table_nw = P.tab_nw;
+    s->st.n_pipes = npipes;
     s->st.Tmax = d.Tmax;
     s->st.kappamax = d.kappamax;
     s->st.Ibmax = d.Ibmax_d;

@@ -120,6 +120,7 @@ typedef struct dnsmc_stats {
     double Ibmax;
     int table_mode;           /* per-step table reads: 0 global, 1 Ib in LDS, 2 kappa+Ib slice in LDS */
     int table_lo, table_nw;   /* temperature nodes held in LDS (mode 2) */
+    int n_pipes;              /* batches in flight at once (one stream each); their traversal launches overlap */
 } dnsmc_stats;
 
 /* Defaults equal to the reference's shipped code/param.h + NarrowBand.h. */
