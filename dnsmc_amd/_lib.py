@@ -59,7 +59,7 @@ class Config(C.Structure):
         ("nsnap", C.c_int),
         ("precision", C.c_int),
         ("tally", C.c_int),
-        ("chunk", C.c_int),
+        ("srt", C.c_int),
         ("batch_rays", C.c_longlong),
         ("table_dir", C.c_char * 512),
     ]
@@ -94,7 +94,7 @@ def lib():
     L.dnsmc_destroy.restype = None
     L.dnsmc_last_error.restype = C.c_char_p
     L.dnsmc_load_tables.argtypes = [vp]
-    L.dnsmc_set_tables.argtypes = [vp, i32, i32, i32] + [f32p] * 8
+    L.dnsmc_set_tables.argtypes = [vp, i32, i32, i32] + [f32p] * 9
     L.dnsmc_set_field.argtypes = [vp, i32, vp, i32]
     L.dnsmc_set_mlp.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_float]
     L.dnsmc_set_emitters_slab.argtypes = [vp, i32]

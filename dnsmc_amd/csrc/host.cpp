@@ -54,7 +54,7 @@ void default_config(dnsmc_config* c)
     c->nsnap = 1;
     c->precision = 0;
     c->tally = DNSMC_TALLY_F64_ATOMIC;
-    c->chunk = 16;
+    c->srt = 1;                                      // code/param.h:47
     c->batch_rays = 1ll << 26;
     snprintf(c->table_dir, sizeof(c->table_dir), "%s", "../distr/tables");
 }
@@ -133,7 +133,7 @@ int parse_config(const char* text, dnsmc_config* c, std::string* err)
             else if (k == "nsnap") c->nsnap = (int)x;
             else if (k == "precision") c->precision = (int)x;
             else if (k == "tally") c->tally = (int)x;
-            else if (k == "chunk") c->chunk = (int)x;
+            else if (k == "srt") c->srt = (int)x;
             else if (k == "batch_rays") c->batch_rays = (long long)x;
             else { *err = "unknown config key '" + k + "'"; return DNSMC_EINVAL; }
         }
@@ -527,6 +527,15 @@ int build_derived(const dnsmc_config& c, const Tables& t, float Tmax, Derived& d
         d.Ibmax = (float)d.Ibmax_d;
         d.g_search = (c.spectral == DNSMC_SPEC_GREY_REF) ? 0 : 1;
         d.u_off = 1.0f;
+        // sorted mode: gas tables only (code/NarrowBand.h:81-90 force srt 0 for grey 1 and 2);
+        // band order by kavg ascending (code/mc_gpu.cu:941-957), ties keep the lower band first
+        if (c.srt == 1 && c.spectral == DNSMC_SPEC_GAS_CK) {
+            d.sorted = 1;
+            d.band_order.resize(nB);
+            for (int b = 0; b < nB; ++b) d.band_order[b] = b;
+            std::stable_sort(d.band_order.begin(), d.band_order.end(),
+                             [&](int x, int y) { return t.kavg[x] < t.kavg[y]; });
+        }
     } else {
         if (!(etot > 0.0)) { *err = "spectral table emits nothing at Tmax"; return DNSMC_EINVAL; }
         double cum = 0.0;

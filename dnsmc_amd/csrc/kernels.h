@@ -127,6 +127,8 @@ struct GenArgs {
     int sub_len;                   // bundles per generator thread
     int n0, n1, n2;                // level-0 sizes (reference stream decomposition)
     int num_streams, num_gpu, gpu, p_row, ystart;
+    const int* band_order;         // sorted mode: band at each position of the kavg order
+    int draws_find;                // sorted mode: XORWOW uniforms per (band, g) draw
     // learned proposal
     MlpWeights mlpw;
     FeatArgs feat;

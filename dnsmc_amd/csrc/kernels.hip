@@ -118,6 +118,38 @@ __device__ __forceinline__ uint32_t sobol_at(const uint32_t* v, uint64_t n)
     return x;
 }
 
+// XORWOW stream of a compat slot (cell, block v), seeded as
+// curand_init(1234, subsequence, 0) is in code/mc_gpu.cu:99: own seed
+// scramble, subsequence = the reference's product
+// (idx+1)(stream+1)(v+1)(1+gpu)ystart over its stream decomposition of the
+// cell (code/mc_gpu.cu:84-87), jump 2^67 per subsequence.
+__device__ Xorwow compat_stream(const GenArgs& a, int cell, int v)
+{
+    int ci = cell % a.n0, cj = (cell / a.n0) % a.n1, ck = cell / (a.n0 * a.n1);
+    int per_stream = a.n0 / a.num_streams;
+    int stream = ci / per_stream;
+    int jloc = cj - (a.ystart - 1);
+    int kloc = ck - a.gpu * (a.n2 / a.num_gpu);
+    int idx = (ci - stream * per_stream) * (a.n2 / a.num_gpu * (a.n1 / a.p_row)) + jloc * (a.n2 / a.num_gpu) + kloc;
+    int32_t prod = (int32_t)((uint32_t)(idx + 1) * (uint32_t)(stream + 1) * (uint32_t)(v + 1) *
+                             (uint32_t)(1 + a.gpu) * (uint32_t)a.ystart);
+    uint64_t subseq = (uint64_t)(int64_t)prod;
+    Xorwow s{a.xw_seed[0], a.xw_seed[1], a.xw_seed[2], a.xw_seed[3], a.xw_seed[4], a.xw_seed[5]};
+    for (int j = 0; j < 60; ++j)
+        if ((subseq >> j) & 1ull) gf2_apply(a.xw_jump_sub + (size_t)j * 800, s);
+    return s;
+}
+
+// Advance an XORWOW state by `off` draws (the Weyl counter d by 362437 per draw).
+__device__ void xorwow_skip(const GenArgs& a, Xorwow& s, uint64_t off)
+{
+    for (int j = 0; j < 64; ++j)
+        if ((off >> j) & 1ull) {
+            gf2_apply(a.xw_jump_off + (size_t)j * 800, s);
+            if (j < 32) s.d += 362437u * (1u << j);
+        }
+}
+
 // One thread per (slot, run of sub_len bundles).  The stream of a slot (cell c,
 // block v) is XORWOW seeded as curand_init(1234, subsequence, 0) is in
 // code/mc_gpu.cu:99 (own seed scramble, subsequence = the reference's product
@@ -147,26 +179,9 @@ __global__ __launch_bounds__(256) void raygen_compat(GenArgs a)
         const int4 si = a.slotinfo[a.slot0 + sl];
         const int cell = si.y;
         const int v = si.x % a.slots_per_emitter;
-        // reference stream decomposition of the cell (code/mc_gpu.cu:84-87)
-        int ci = cell % a.n0, cj = (cell / a.n0) % a.n1, ck = cell / (a.n0 * a.n1);
-        int per_stream = a.n0 / a.num_streams;
-        int stream = ci / per_stream;
-        int jloc = cj - (a.ystart - 1);
-        int kloc = ck - a.gpu * (a.n2 / a.num_gpu);
-        int idx = (ci - stream * per_stream) * (a.n2 / a.num_gpu * (a.n1 / a.p_row)) + jloc * (a.n2 / a.num_gpu) + kloc;
-        int32_t prod = (int32_t)((uint32_t)(idx + 1) * (uint32_t)(stream + 1) * (uint32_t)(v + 1) *
-                                 (uint32_t)(1 + a.gpu) * (uint32_t)a.ystart);
-        uint64_t subseq = (uint64_t)(int64_t)prod;
-        Xorwow s{a.xw_seed[0], a.xw_seed[1], a.xw_seed[2], a.xw_seed[3], a.xw_seed[4], a.xw_seed[5]};
-        for (int j = 0; j < 60; ++j)
-            if ((subseq >> j) & 1ull) gf2_apply(a.xw_jump_sub + (size_t)j * 800, s);
+        Xorwow s = compat_stream(a, cell, v);
         int h0 = run * a.sub_len;
-        uint64_t off = (uint64_t)h0 * (uint64_t)a.draws_per_bundle;
-        for (int j = 0; j < 64; ++j)
-            if ((off >> j) & 1ull) {
-                gf2_apply(a.xw_jump_off + (size_t)j * 800, s);
-                if (j < 32) s.d += 362437u * (1u << j);
-            }
+        xorwow_skip(a, s, (uint64_t)h0 * (uint64_t)a.draws_per_bundle);
         int h1 = h0 + a.sub_len < a.rays_per_slot ? h0 + a.sub_len : a.rays_per_slot;
         uint64_t n = (uint64_t)v * (uint64_t)a.rays_per_slot + (uint64_t)h0;
         uint32_t sx0 = sobol_at(s_sob, n), sx1 = sobol_at(s_sob + 32, n);
@@ -198,6 +213,140 @@ __global__ __launch_bounds__(256) void raygen_compat(GenArgs a)
             a.w[q] = w;
             a.winv[q] = winv;
         }
+    }
+}
+
+// Sorted compat generation (srt == 1: code/mc_gpu.cu:100-102, :138-139).  The
+// reference draws all photmax (band, g) pairs of the slot's stream first and
+// counts them (kernel_find, :786-812), then replays the counts position by
+// position of the kavg band order, g ascending inside a band (wave_find,
+// :897-930), drawing each bundle's direction in the photon loop.  Bundles with
+// the same (band, g) are interchangeable until a direction is attached, so
+// the replay is a histogram and a scan: one block per slot,
+//  (1) each thread jumps to its run of (band, g) draws and counts them in LDS,
+//  (2) the histogram is scanned in replay order (position p, then g),
+//  (3) bundle h takes the bin where the running count first exceeds h, and its
+//      direction is Sobol point v*photmax + h, or XORWOW uniforms
+//      draws_find*photmax + 2h (phi) and + 2h + 1 (theta) of the stream.
+// The reference permutes its tables into the sorted order, so position p
+// carries band band_order[p]; the descriptor holds that band.
+__global__ __launch_bounds__(256) void raygen_compat_sorted(GenArgs a)
+{
+    extern __shared__ float lds[];
+    __shared__ uint32_t s_part[256];
+    const int nbin = a.nB * a.nQ;
+    float* s_cdfb = lds;
+    float* s_cdfg = s_cdfb + a.nB;
+    uint32_t* s_sob = (uint32_t*)(s_cdfg + nbin);
+    int* s_ord = (int*)(s_sob + 64);
+    uint32_t* s_hist = (uint32_t*)(s_ord + a.nB);
+    uint32_t* s_cum = s_hist + nbin;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (int i = tid; i < a.nB; i += nth) {
+        s_cdfb[i] = a.cdf_band[i];
+        s_ord[i] = a.band_order[i];
+    }
+    for (int i = tid; i < nbin; i += nth) {
+        s_cdfg[i] = a.cdf_g[i];
+        s_hist[i] = 0u;
+    }
+    for (int i = tid; i < 64; i += nth) s_sob[i] = a.sobol_v[i];
+    __syncthreads();
+    const int R = a.rays_per_slot;
+    const int L = (R + nth - 1) / nth;
+    const int h0 = tid * L < R ? tid * L : R;
+    const int h1 = h0 + L < R ? h0 + L : R;
+    for (long long sl = blockIdx.x; sl < a.nslots; sl += gridDim.x) {
+        const int4 si = a.slotinfo[a.slot0 + sl];
+        const int cell = si.y;
+        const int v = si.x % a.slots_per_emitter;
+        const Xorwow s0 = compat_stream(a, cell, v);
+        // (1) histogram of the (band, g) draws h0..h1-1
+        if (h0 < h1) {
+            Xorwow s = s0;
+            xorwow_skip(a, s, (uint64_t)h0 * (uint64_t)a.draws_find);
+            for (int h = h0; h < h1; ++h) {
+                int band = cdf_search(s_cdfb, a.nB, u01_incl(xorwow_next(s)));
+                int g = 0;
+                if (a.g_search) g = cdf_search(s_cdfg + band * a.nQ, a.nQ, u01_incl(xorwow_next(s)));
+                g = g > a.nQ - 1 ? a.nQ - 1 : g;
+                atomicAdd(&s_hist[band * a.nQ + g], 1u);
+            }
+        }
+        __syncthreads();
+        // (2) inclusive scan in replay order: entry k = p*nQ + g counts bin order[p]*nQ + g
+        const int per = (nbin + nth - 1) / nth;
+        const int k0 = tid * per < nbin ? tid * per : nbin;
+        const int k1 = k0 + per < nbin ? k0 + per : nbin;
+        uint32_t run = 0;
+        for (int k = k0; k < k1; ++k) {
+            const int p = k / a.nQ, g = k - p * a.nQ;
+            run += s_hist[s_ord[p] * a.nQ + g];
+        }
+        s_part[tid] = run;
+        __syncthreads();
+        for (int o = 1; o < nth; o <<= 1) {
+            uint32_t add = tid >= o ? s_part[tid - o] : 0u;
+            __syncthreads();
+            s_part[tid] += add;
+            __syncthreads();
+        }
+        run = tid > 0 ? s_part[tid - 1] : 0u;
+        for (int k = k0; k < k1; ++k) {
+            const int p = k / a.nQ, g = k - p * a.nQ;
+            run += s_hist[s_ord[p] * a.nQ + g];
+            s_cum[k] = run;
+        }
+        __syncthreads();
+        // (3) bundles h0..h1-1
+        if (h0 < h1) {
+            int lo = 0, hi = nbin - 1;             // first k with s_cum[k] > h0 (s_cum[nbin-1] == R)
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_cum[mid] > (uint32_t)h0) hi = mid; else lo = mid + 1;
+            }
+            int k = lo;
+            Xorwow s = s0;
+            uint64_t n = (uint64_t)v * (uint64_t)R + (uint64_t)h0;
+            uint32_t sx0 = 0, sx1 = 0;
+            if (a.use_sobol) {
+                sx0 = sobol_at(s_sob, n);
+                sx1 = sobol_at(s_sob + 32, n);
+            } else {
+                xorwow_skip(a, s, (uint64_t)R * (uint64_t)a.draws_find + 2ull * (uint64_t)h0);
+            }
+            const float Ti = a.T0[(size_t)a.snap * a.tstride0 + si.z];
+            for (int h = h0; h < h1; ++h) {
+                uint32_t mp, mt;
+                if (a.use_sobol) {
+                    mp = sx0 >> 8;
+                    mt = sx1 >> 8;
+                    int c = __builtin_ctzll(~n);
+                    sx0 ^= s_sob[c];
+                    sx1 ^= s_sob[32 + c];
+                    ++n;
+                } else {
+                    mp = xorwow_next(s) >> 8;
+                    mt = xorwow_next(s) >> 8;
+                }
+                while (s_cum[k] <= (uint32_t)h) ++k;
+                const int p = k / a.nQ, g = k - p * a.nQ;
+                const int band = s_ord[p];
+                float w, winv;
+                bundle_weight(a, a.Ib, Ti, band, g, &w, &winv);
+                uint32_t w0, w1;
+                desc_pack(mt, mp, band, g, &w0, &w1);
+                const long long q = sl * R + h;
+                a.w0[q] = w0;
+                a.w1[q] = w1;
+                a.w[q] = w;
+                a.winv[q] = winv;
+            }
+        }
+        // the next slot reuses the histogram
+        __syncthreads();
+        for (int i = tid; i < nbin; i += nth) s_hist[i] = 0u;
+        __syncthreads();
     }
 }
 

@@ -29,6 +29,7 @@ __global__ void trace(TraceArgs P);
 __global__ void t_range(const float* T, int n0, int n1, int n2, int nbx, int nby, unsigned* lohi);
 __global__ void raygen_plain(GenArgs a);
 __global__ void raygen_compat(GenArgs a);
+__global__ void raygen_compat_sorted(GenArgs a);
 __global__ void finalize(const double* E, const double* A, const double* Q2, long long nemit, int spe, int R,
                          int compat, double* Q, double* Eo, double* Ao, double* sd);
 __global__ void spline_pass(SplineArgs s);
@@ -138,6 +139,7 @@ struct dnsmc_solver {
     bool spline_ready = false;
     // tables
     DevBuf<float> d_kq, d_Ib, d_Ibw, d_cdfb, d_cdfg, d_inv_emax;
+    DevBuf<int> d_band_order;   // sorted compat mode
     DevBuf<double> d_inv_emax_d;
     // compat
     DevBuf<uint32_t> d_xw_sub, d_xw_off, d_sobol;
@@ -384,6 +386,7 @@ static int ensure_derived(dnsmc_solver* s, float Tmax)
     HIPCHK(s->d_cdfb.upload(d.cdf_band, s->stream));
     HIPCHK(s->d_cdfg.upload(d.cdf_g, s->stream));
     HIPCHK(s->d_inv_emax.upload(d.inv_emax, s->stream));
+    if (d.sorted) HIPCHK(s->d_band_order.upload(d.band_order, s->stream));
     {   // plain probability of each (band, g) bin: the CDF steps the plain draw uses
         std::vector<float> pbg((size_t)d.nB * d.nQ, 0.f);
         double pb0 = 0.0;
@@ -779,9 +782,15 @@ static int do_run(dnsmc_solver* s)
         G.n0 = c.n[0]; G.n1 = c.n[1]; G.n2 = c.n[2];
         G.num_streams = c.num_streams; G.num_gpu = c.num_gpu; G.gpu = c.gpu; G.p_row = c.p_row;
         G.ystart = s->ystart;
+        G.band_order = s->d_band_order.p;
+        G.draws_find = 1 + (d.g_search ? 1 : 0);
     }
     const bool gib = spec == kSpecTable && (size_t)d.nB * d.nT <= 16384;
     const size_t lds_gen = sizeof(float) * ((size_t)d.nB + (size_t)d.nB * d.nQ + (gib ? (size_t)d.nB * d.nT : 0) + 64);
+    const bool sorted = c.mode == DNSMC_MODE_COMPAT && d.sorted;
+    const size_t lds_sorted = sizeof(float) * (2 * (size_t)d.nB + 3 * (size_t)d.nB * d.nQ + 64);
+    if (sorted && lds_sorted + 1024 > s->lds_max)
+        return fail(DNSMC_EINVAL, "sorted mode: too many (band, g) bins for the per-block histogram");
     size_t lds_learn = 0;
     if (c.mode == DNSMC_MODE_LEARNED) {
         if (!s->have_mlp) return fail(DNSMC_ESTATE, "learned mode needs dnsmc_set_mlp first");
@@ -818,6 +827,9 @@ static int do_run(dnsmc_solver* s)
                 nb = (int)std::min<long long>((nsl + 7) / 8, (long long)s->ncu * 3);
                 hipLaunchKernelGGL(raygen_learned, dim3(nb), dim3(256), lds_learn, st, G, (const float*)pp.d_qcdf.p,
                                    (const float*)s->d_pcdf.p);
+            } else if (sorted) {
+                int nb = (int)std::min<long long>(nsl, (long long)s->ncu * 8);
+                hipLaunchKernelGGL(raygen_compat_sorted, dim3(nb), dim3(256), lds_sorted, st, G);
             } else if (c.mode == DNSMC_MODE_COMPAT) {
                 long long nth = nsl * ((R + G.sub_len - 1) / G.sub_len);
                 int nb = (int)std::min<long long>((nth + 255) / 256, 1 << 20);
@@ -1018,7 +1030,8 @@ int dnsmc_load_tables(dnsmc_solver* s)
 }
 
 int dnsmc_set_tables(dnsmc_solver* s, int nB, int nQ, int nT, const float* Tnb, const float* kP, const float* wvl,
-                     const float* wvr, const float* wvc, const float* kq, const float* prob, const float* probg)
+                     const float* wvr, const float* wvc, const float* kq, const float* prob, const float* probg,
+                     const float* kavg)
 {
     if (!s || nB < 1 || nQ < 1 || nT < 2 || !Tnb || !kP || !wvl || !wvr || !wvc || !kq || !prob || !probg)
         return fail(DNSMC_EINVAL, "dnsmc_set_tables: bad arguments");
@@ -1035,7 +1048,15 @@ int dnsmc_set_tables(dnsmc_solver* s, int nB, int nQ, int nT, const float* Tnb, 
     t.prob.assign(prob, prob + (size_t)nT * nB);
     t.probg.assign(probg, probg + (size_t)nT * nB * nQ);
     t.kavg.assign(nB, 0.f);
-    for (int b = 0; b < nB; ++b) t.kavg[b] = t.kq[((size_t)b * nT + nT - 1) * nQ];
+    for (int b = 0; b < nB; ++b) {
+        if (kavg) {
+            t.kavg[b] = kavg[b];
+        } else {
+            double m = 0.0;
+            for (int g = 0; g < nQ; ++g) m += t.kq[((size_t)b * nT + nT - 1) * nQ + g];
+            t.kavg[b] = (float)(m / nQ);
+        }
+    }
     s->have_tables = true;
     s->der_Tmax = -1.f;
     return DNSMC_OK;

@@ -42,6 +42,8 @@ struct Derived {
     double Ibmax_d = 0.0;
     int g_search = 1;
     float u_off = 0.5f;
+    int sorted = 0;              // compat srt == 1 on gas tables
+    std::vector<int> band_order; // sorted: band at each position of the ascending-kavg order
 };
 
 // Host-side builders (host.cpp).

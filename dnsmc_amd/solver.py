@@ -90,13 +90,16 @@ class Solver:
         check(lib().dnsmc_load_tables(self.h))
 
     def set_tables(self, t):
-        """t: dict with Tnb, kP, wvl, wvr, wvc, kq[nB][nT][nQ], prob[nT][nB], probg[nT][nB][nQ]."""
+        """t: dict with Tnb, kP, wvl, wvr, wvc, kq[nB][nT][nQ], prob[nT][nB], probg[nT][nB][nQ] and
+        optionally kavg[nB] (the sorted mode's band key)."""
         arrs = {k: np.ascontiguousarray(t[k], np.float32) for k in
                 ("Tnb", "kP", "wvl", "wvr", "wvc", "kq", "prob", "probg")}
+        kavg = np.ascontiguousarray(t["kavg"], np.float32) if t.get("kavg") is not None else None
         nB, nT, nQ = arrs["kq"].shape
-        self._tables = arrs
+        self._tables = dict(arrs, kavg=kavg)
         check(lib().dnsmc_set_tables(self.h, nB, nQ, nT, *[_f32(arrs[k]) for k in
-                                                            ("Tnb", "kP", "wvl", "wvr", "wvc", "kq", "prob", "probg")]))
+                                                            ("Tnb", "kP", "wvl", "wvr", "wvc", "kq", "prob", "probg")],
+                                     _f32(kavg) if kavg is not None else None))
 
     # field / emitters -----------------------------------------------------
     def set_field(self, snap, T_ghost):

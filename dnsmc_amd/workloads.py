@@ -30,7 +30,7 @@ def channel_config(n, rays_per_cell=100, spectral=0, Tw=573.0, table_dir="", nsn
         bc=((1, 1), (2, 2), (1, 1)), Tw=((0.0, 0.0), (Tw, Tw), (0.0, 0.0)), polar_axis=1,
         nvar=3, photmax=20000, rays_per_cell=rays_per_cell, grid_num=1, maxs=(100000, 100000, 100000, 100000, 100000, 100000),
         toll=1.0e-3, spectral=spectral, kappa_grey=1.0, mode=0, sobol=1, num_streams=16, num_gpu=1, gpu=0,
-        p_row=1, seed=20240601, nsnap=nsnap, precision=0, tally=0, chunk=16, batch_rays=1 << 30,
+        p_row=1, seed=20240601, nsnap=nsnap, precision=0, tally=0, srt=0, batch_rays=1 << 30,
         table_dir=table_dir,
     )
     d.update(kw)
@@ -169,8 +169,11 @@ def write_reference_tables(t, dirname):
 
 
 def table_arrays(t):
-    """Dict accepted by Solver.set_tables (float32 arrays in reference layouts)."""
-    return {k: np.asarray(t[k], np.float32) for k in ("Tnb", "kP", "wvl", "wvr", "wvc", "kq", "prob", "probg")}
+    """Dict accepted by Solver.set_tables (float32 arrays in reference layouts);
+    kavg is the last temperature row's value, as NarrBand%d.txt holds it."""
+    d = {k: np.asarray(t[k], np.float32) for k in ("Tnb", "kP", "wvl", "wvr", "wvc", "kq", "prob", "probg")}
+    d["kavg"] = np.asarray(t["kavg"][:, -1], np.float32)
+    return d
 
 
 WORKLOADS = {

@@ -100,7 +100,8 @@ typedef struct dnsmc_config {
     int nsnap;               /* temperature snapshots batched in one run */
     int precision;           /* 0 FP32 traversal, 1 FP64 traversal */
     int tally;               /* DNSMC_TALLY_* */
-    int chunk;               /* reserved (accepted, unused) */
+    int srt;                 /* compat mode, gas tables: replay each block's drawn (band, g) multiset in
+                                ascending-kavg band order (code/param.h:47 srt; forced off for grey 1/2) */
     long long batch_rays;    /* rays generated per traversal launch */
     char table_dir[512];     /* directory of the spectral tables */
 } dnsmc_config;
@@ -137,11 +138,13 @@ const char *dnsmc_last_error(void);
  * cfg->table_dir (code/read_tables.cpp:20-92 gas, :187-241 particle;
  * grey==1 and grey-T4 tables are synthesised).  dnsmc_set_tables takes the
  * same data as arrays: Tnb[nT], kP[nT], wvl/wvr/wvc[nB], kq[nB][nT][nQ],
- * prob[nT][nB], probg[nT][nB][nQ], gq/wq[nQ] (gq, wq may be NULL). */
+ * prob[nT][nB], probg[nT][nB][nQ], and kavg[nB], the band's kavg column of
+ * its last temperature row in NarrBand%d.txt (the sorted mode's band key,
+ * code/read_tables.cpp:50; NULL: the mean over g of kq at the last row). */
 int dnsmc_load_tables(dnsmc_solver *s);
 int dnsmc_set_tables(dnsmc_solver *s, int nB, int nQ, int nT, const float *Tnb, const float *kP,
                      const float *wvl, const float *wvr, const float *wvc, const float *kq,
-                     const float *prob, const float *probg);
+                     const float *prob, const float *probg, const float *kavg);
 
 /* Temperature field of snapshot `snap`, Fortran layout with ghosts
  * (n0+2)(n1+2)(n2+2), host or device pointer (is_device). */

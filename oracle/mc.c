@@ -346,6 +346,79 @@ void orc_gen_compat(const orc_problem *P, uint64_t seed, uint64_t subseq, int us
     }
 }
 
+/* Band order of the sorted mode (code/mc_gpu.cu:941-957 sort_idx): order[p] is
+ * the band at position p when the bands are ranked by kavg ascending.  The
+ * reference uses std::sort (ties in unspecified order); ties here keep the
+ * lower band index first. */
+void orc_sort_idx(const float *kavg, int nB, int *order)
+{
+    for (int b = 0; b < nB; ++b) order[b] = b;
+    for (int i = 1; i < nB; ++i) {          /* insertion sort: stable */
+        int x = order[i], j = i - 1;
+        while (j >= 0 && kavg[order[j]] > kavg[x]) { order[j + 1] = order[j]; --j; }
+        order[j + 1] = x;
+    }
+}
+
+/* Sorted compat generation (srt == 1; code/mc_gpu.cu:99-102, :138-139,
+ * kernel_find :786-812, wave_find :897-930).  kernel_find draws nr (band, g)
+ * pairs from the slot's XORWOW stream with find_band (Rwave, then Rwave2 when
+ * g is searched) and counts them per band (nb_cnt) and per (g, band) (g_cnt).
+ * The photon loop then draws each bundle's direction (Sobol point, or two
+ * more XORWOW uniforms phi then theta) and takes its (band, g) from wave_find:
+ * positions p = 0.. of the kavg order, each band's count replayed with g
+ * ascending.  The reference's tables are permuted into that order, so the
+ * bundle at position p carries band order[p]. */
+void orc_gen_compat_sorted(const orc_problem *P, uint64_t seed, uint64_t subseq, int use_sobol,
+                           uint64_t sobol_offset, int cell, int nr, const int *order, orc_ray_desc *out)
+{
+    orc_xorwow st;
+    orc_xorwow_init(&st, seed, subseq, 0);
+    uint32_t v0[32], v1[32];
+    orc_sobol_dirs(0, v0);
+    orc_sobol_dirs(1, v1);
+    const int nB = P->nB, nQ = P->nQ;
+    int *nb_cnt = (int *)calloc((size_t)nB, sizeof(int));
+    int *g_cnt = (int *)calloc((size_t)nB * nQ, sizeof(int));    /* [g][band] */
+    for (int h = 0; h < nr; ++h) {                               /* kernel_find */
+        int band = orc_cdf_search(P->cdf_band, nB, orc_u01_incl(orc_xorwow_next(&st)));
+        int g = 0;
+        if (P->g_search) g = sample_g(P, band, orc_u01_incl(orc_xorwow_next(&st)));
+        if (g > nQ - 1) g = nQ - 1;
+        nb_cnt[band] += 1;
+        g_cnt[(size_t)g * nB + band] += 1;
+    }
+    float Ti = P->T[0][cell];
+    int nb = 0, g = 0, countnb = 0, countg = 0;
+    for (int h = 0; h < nr; ++h) {
+        uint32_t mp, mt;
+        if (use_sobol) {
+            mp = orc_sobol_point(v0, sobol_offset + (uint64_t)h) >> 8;
+            mt = orc_sobol_point(v1, sobol_offset + (uint64_t)h) >> 8;
+        } else {
+            mp = orc_xorwow_next(&st) >> 8;
+            mt = orc_xorwow_next(&st) >> 8;
+        }
+        /* wave_find */
+        int more = 1;
+        while (more) {
+            if (countnb < nb_cnt[order[nb]]) { countnb += 1; more = 0; }
+            else { nb += 1; countnb = 0; countg = 0; g = 0; more = 1; }
+        }
+        while (!more) {
+            if (countg < g_cnt[(size_t)g * nB + order[nb]]) { countg += 1; more = 1; }
+            else { g += 1; countg = 0; more = 0; }
+        }
+        int band = order[nb];
+        int bg = band * nQ + g;
+        float w, winv;
+        weights_f32(P, Ti, bg, band, &w, &winv);
+        desc_pack(&out[h], band, g, mt, mp, w, winv);
+    }
+    free(nb_cnt);
+    free(g_cnt);
+}
+
 void orc_trace_descs(const orc_problem *P, int cell, const orc_ray_desc *d, int nr,
                      double *E, double *A, double *Q2, long *nsteps)
 {

@@ -156,6 +156,11 @@ void orc_gen_philox(const orc_problem *P, uint64_t seed, uint32_t snap, int cell
  * 0/1 at points v*photmax+h for directions when sobol==1 (mc_gpu.cu:93-144). */
 void orc_gen_compat(const orc_problem *P, uint64_t seed, uint64_t subseq, int use_sobol,
                     uint64_t sobol_offset, int cell, int nr, orc_ray_desc *out);
+/* Sorted mode (srt == 1): band order by kavg (sort_idx, mc_gpu.cu:941-957) and
+ * the count-and-replay generation of kernel_find / wave_find (:786-812, :897-930). */
+void orc_sort_idx(const float *kavg, int nB, int *order);
+void orc_gen_compat_sorted(const orc_problem *P, uint64_t seed, uint64_t subseq, int use_sobol,
+                           uint64_t sobol_offset, int cell, int nr, const int *order, orc_ray_desc *out);
 
 /* Decode helpers. */
 void orc_desc_decode(const orc_ray_desc *d, int nQ, int *band, int *g, float *u_theta, float *u_phi);

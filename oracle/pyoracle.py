@@ -106,6 +106,9 @@ class Oracle:
                                      C.c_int, C.c_void_p]
         L.orc_gen_compat.argtypes = [C.POINTER(OrcProblem), C.c_uint64, C.c_uint64, C.c_int, C.c_uint64,
                                      C.c_int, C.c_int, C.c_void_p]
+        L.orc_gen_compat_sorted.argtypes = [C.POINTER(OrcProblem), C.c_uint64, C.c_uint64, C.c_int, C.c_uint64,
+                                            C.c_int, C.c_int, _i32p, C.c_void_p]
+        L.orc_sort_idx.argtypes = [_f32p, C.c_int, _i32p]
         L.orc_trace_descs.argtypes = [C.POINTER(OrcProblem), C.c_int, C.c_void_p, C.c_int, _f64p, _f64p,
                                       _f64p, C.POINTER(C.c_long)]
         for fn in (L.orc_mc_philox, L.orc_mc_philox_f64):
@@ -280,6 +283,19 @@ class Oracle:
         self.lib.orc_gen_compat(C.byref(P), seed, subseq, int(use_sobol), sobol_offset, cell, nr,
                                 out.ctypes.data)
         return out
+
+    def gen_compat_sorted(self, P, seed, subseq, use_sobol, sobol_offset, cell, nr, order):
+        out = np.zeros(nr, RAY_DESC)
+        order = np.ascontiguousarray(order, np.int32)
+        self.lib.orc_gen_compat_sorted(C.byref(P), seed, subseq, int(use_sobol), sobol_offset, cell, nr,
+                                       _ptr(order, _i32p), out.ctypes.data)
+        return out
+
+    def sort_idx(self, kavg):
+        kavg = np.ascontiguousarray(kavg, np.float32)
+        order = np.zeros(len(kavg), np.int32)
+        self.lib.orc_sort_idx(_ptr(kavg, _f32p), len(kavg), _ptr(order, _i32p))
+        return order
 
     def trace_descs(self, P, cell, descs):
         d = np.ascontiguousarray(descs, RAY_DESC)

@@ -95,7 +95,7 @@ def read_reference_tables(dirname, gas=True):
     nB = 0
     while os.path.exists(os.path.join(dirname, "NarrBand%d.txt" % nB)):
         nB += 1
-    wvl, wvr, wvc, kq = [], [], [], []
+    wvl, wvr, wvc, kq, kavg = [], [], [], [], []
     for b in range(nB):
         with open(os.path.join(dirname, "NarrBand%d.txt" % b)) as f:
             head = [float(x) for x in f.readline().split()]
@@ -103,8 +103,10 @@ def read_reference_tables(dirname, gas=True):
             if gas:
                 rows = np.array([[float(x) for x in f.readline().split()] for _ in range(nT)])
                 kq.append(rows[:, 3:])
+                kavg.append(rows[-1, 2])          # the last row's value is kept (code/read_tables.cpp:50)
             else:
                 kq.append(np.full((nT, 1), head[3]))
+                kavg.append(head[3])
     kq = np.array(kq)                       # [b][t][g]
     nQ = kq.shape[2]
     p = np.loadtxt(os.path.join(dirname, "prob_new2.txt" if gas else "prob.txt"), ndmin=2)
@@ -114,7 +116,7 @@ def read_reference_tables(dirname, gas=True):
     else:
         probg = np.zeros((nT, nB, 1))
     return {k: np.asarray(v, np.float32) for k, v in
-            dict(Tnb=Tnb, kP=kP, wvl=wvl, wvr=wvr, wvc=wvc, kq=kq, prob=prob, probg=probg).items()}
+            dict(Tnb=Tnb, kP=kP, wvl=wvl, wvr=wvr, wvc=wvc, kq=kq, prob=prob, probg=probg, kavg=kavg).items()}
 
 
 def read_resfortm():

@@ -260,3 +260,51 @@ def test_table_placement_bit_identical(ck_table_dir, monkeypatch):
     assert modes[0] == 2 and modes[1] == 0
     np.testing.assert_array_equal(out[0]["E"], out[1]["E"])
     np.testing.assert_array_equal(out[0]["A"], out[1]["A"])
+
+
+@pytest.mark.parametrize("sobol", [1, 0])
+def test_compat_sorted_gas_parity(oracle, ck_table_dir, sobol):
+    """Sorted compat mode (srt == 1) on the correlated-k table: every bundle
+    descriptor bit-exact against the oracle's kernel_find / wave_find replay,
+    per-ray traversal integers on a sample, and the block-averaged tallies."""
+    tdir, _ = ck_table_dir
+    n = (16, 8, 6)
+    R, nvar = 400, 3
+    cfg = workloads.channel_config(n, spectral=0, Tw=573.0, table_dir=tdir, mode=1, srt=1, sobol=sobol,
+                                   photmax=R, nvar=nvar, num_streams=16)
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    S.set_field(0, workloads.ghosted(workloads.field_profile_noise(7, n)))
+    S.set_record(True)
+    S.run()
+    S.wait()
+    d = oracle_problem_dict(S, cfg)
+    P = oracle.problem(d)
+    order = oracle.sort_idx(read_reference_tables(tdir, gas=True)["kavg"])
+    rays = S.rays(S.nemit * nvar * R)
+    rec = S.records(S.nemit * nvar * R)
+    res = S.results(0)
+    for cell in [0, 1, 37, 200, 511, 767]:
+        i, j, k = cell % n[0], (cell // n[0]) % n[1], cell // (n[0] * n[1])
+        dq = []
+        for v in range(nvar):
+            ref = oracle.gen_compat_sorted(P, cfg.seed, ref_subsequence(i, j, k, v, n), sobol, v * R, cell, R, order)
+            q0 = (cell * nvar + v) * R
+            got = rays[q0:q0 + R]
+            np.testing.assert_array_equal(got["w0"], ref["w0"])
+            np.testing.assert_array_equal(got["w1"], ref["w1"])
+            np.testing.assert_allclose(got["w"], ref["w"], rtol=2e-7)
+            band, g = _decode(ref, d["nQ"])
+            for r in range(0, R, 23):
+                ut = (float(ref["w0"][r] & 0xFFFFFF) + d["u_off"]) * 2.0 ** -24
+                up = (float(ref["w1"][r] & 0xFFFFFF) + d["u_off"]) * 2.0 ** -24
+                o = oracle.trace_ray(P, i + 1, j + 1, k + 1, ut, up, int(band[r]), int(g[r]))
+                assert rec["steps"][q0 + r] == o.steps
+                assert rec["end"][q0 + r] == o.end_type
+                assert rec["hash"][q0 + r] == o.hash
+            E, A, _, _ = oracle.trace_descs(P, cell, ref)
+            dq.append((E, A))
+        E = np.mean([e for e, _ in dq])
+        A = np.mean([a for _, a in dq])
+        assert abs(res["E"][cell] - E) <= 1e-5 * abs(E)
+        assert abs(res["Q"][cell] - (E - A)) <= 1e-5 * abs(E)
