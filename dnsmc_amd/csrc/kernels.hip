@@ -750,6 +750,217 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     if (lane == 0) atomicAdd(P.nsteps, st);
 }
 
+// ------------------------------------------------------------ trace_f64 ----
+// FP64 traversal (precision 1, plain estimator): every ray in double, as the
+// FP64 oracle computes it.  Direction from the descriptor's 24-bit uniforms,
+// table lookups interpolated in FP64 from the FP32 tables, FP64 optical depth,
+// transmissivity and gathered emission, bundle weights recomputed in FP64:
+// w = Ibmax (k_i Ib_i) / (k_max Ib_max), E = w D, A = w G / Ib_i.  One thread
+// per ray (grid-stride); each ray's (E, A) goes to pray64 for reduce_rays64.
+__device__ __forceinline__ void lookup_f64(const TraceArgs& P, double Tj, int bg, int band, double* kap, double* ib)
+{
+    double pos = Tj * (double)P.inv_dT + (double)P.mT0;
+    pos = pos < 0.0 ? 0.0 : pos;
+    pos = pos > (double)(P.nT - 1) ? (double)(P.nT - 1) : pos;
+    int t0 = (int)pos;
+    t0 = t0 > P.nT - 2 ? P.nT - 2 : t0;
+    const double fr = pos - (double)t0;
+    const float* kr = P.kq + (size_t)bg * P.nT;
+    const float* ir = P.Ib + (size_t)band * P.nT;
+    *kap = (double)kr[t0] + fr * ((double)kr[t0 + 1] - (double)kr[t0]);
+    *ib = (double)ir[t0] + fr * ((double)ir[t0 + 1] - (double)ir[t0]);
+}
+
+template <int SPEC, bool REC>
+__global__ __launch_bounds__(256) void trace_f64(TraceArgs P, double2* pray64)
+{
+    extern __shared__ float lds[];
+    int* s_lev = (int*)lds;
+    float* s_face = lds + 6 * kLevWords;
+    int* s_off = (int*)(s_face + P.nfaces);
+    float* s_cent = (float*)(s_off + P.noffs);
+    float* s_ibw = s_cent + P.ncent;
+    for (int i = threadIdx.x; i < P.nlev; i += blockDim.x) {
+        int* L = s_lev + i * kLevWords;
+        L[0] = P.lev[i].n[0]; L[1] = P.lev[i].n[1]; L[2] = P.lev[i].n[2];
+        L[3] = P.lev[i].foff[0]; L[4] = P.lev[i].foff[1]; L[5] = P.lev[i].foff[2];
+        L[6] = P.lev[i].maxs;
+        L[11] = P.lev[i].ooff[0]; L[12] = P.lev[i].ooff[1]; L[13] = P.lev[i].ooff[2];
+    }
+    for (int i = threadIdx.x; i < P.nfaces; i += blockDim.x) s_face[i] = P.faces[i];
+    for (int i = threadIdx.x; i < P.noffs; i += blockDim.x) s_off[i] = P.offs[i];
+    for (int i = threadIdx.x; i < P.ncent; i += blockDim.x) s_cent[i] = P.centres[i];
+    for (int i = threadIdx.x; i < 6 * P.nB; i += blockDim.x) s_ibw[i] = P.Ibw[i];
+    __syncthreads();
+    const double Tm = (double)P.Tmax;
+    unsigned long long steps_t = 0;
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < P.nrays;
+         q += (long long)gridDim.x * blockDim.x) {
+        const uint32_t a0 = P.w0[q], a1 = P.w1[q];
+        const int4 si = P.slotinfo[P.slot0 + fdiv((uint32_t)q, P.fd_R)];
+        const int band = desc_band(a0, a1), g = desc_g(a1);
+        const int bg = band * P.nQ + g;
+        const double ut = ((double)(a0 & 0xFFFFFFu) + (double)P.u_off) * 5.9604644775390625e-08;
+        const double up = ((double)(a1 & 0xFFFFFFu) + (double)P.u_off) * 5.9604644775390625e-08;
+        double s[3], inv[3], p[3];
+        {
+            const int a = P.polar;
+            const double cth = 1.0 - 2.0 * ut, st = 2.0 * sqrt(ut * (1.0 - ut));
+            const double ph = 6.283185307179586476925 * up;
+            const double c1 = st * cos(ph), c2 = st * sin(ph);
+            s[0] = a == 0 ? cth : (a == 1 ? c2 : c1);
+            s[1] = a == 0 ? c1 : (a == 1 ? cth : c2);
+            s[2] = a == 0 ? c2 : (a == 1 ? c1 : cth);
+        }
+        int f[3], c[3];
+        for (int a = 0; a < 3; ++a) {
+            if (s[a] == 0.0) s[a] = 1e-10;
+            f[a] = s[a] < 0.0;
+            inv[a] = 1.0 / s[a];
+        }
+        c[0] = (si.w & 1023) + 1;
+        c[1] = ((si.w >> 10) & 1023) + 1;
+        c[2] = ((si.w >> 20) & 1023) + 1;
+        for (int a = 0; a < 3; ++a) p[a] = (double)s_cent[P.coff[a] + c[a]];
+        const double Ti = (double)P.T[0][(size_t)P.snap * P.lev[0].tstride + si.z];
+        double ki, ii, kmx, imx;
+        if (SPEC == kSpecGreyT4) {
+            ki = kmx = (double)P.kappa_grey;
+            ii = Ti * Ti * Ti * Ti;
+            imx = Tm * Tm * Tm * Tm;
+        } else {
+            lookup_f64(P, Ti, bg, band, &ki, &ii);
+            lookup_f64(P, Tm, bg, band, &kmx, &imx);
+        }
+        const double w = P.Ibmax_d * (ki * ii) / (kmx * imx);
+        double tau = 0.0, tra = 1.0, G = 0.0, D = 1.0;
+        const double tau_cut = (double)P.tau_cut;
+        int steps = 0, endt = 2;
+        uint32_t hash = 2166136261u;
+        bool done = false;
+        for (int lev = 0; lev < P.nlev && !done; ++lev) {
+            const int* L = s_lev + lev * kLevWords;
+            const int n[3] = {L[0], L[1], L[2]};
+            const int fo[3] = {L[3], L[4], L[5]};
+            const int oo[3] = {L[11], L[12], L[13]};
+            const float* T = P.T[lev] + (size_t)P.snap * P.lev[lev].tstride;
+            if (lev > 0) {
+                const int* Lp = s_lev + (lev - 1) * kLevWords;
+                for (int a = 0; a < 3; ++a) {
+                    const int r = Lp[a] / n[a];
+                    c[a] = (c[a] + r - 1) / r;
+                }
+            }
+            int cnt = 0;
+            while (cnt < L[6]) {
+                double fx[3], d[3];
+                for (int a = 0; a < 3; ++a) {
+                    fx[a] = (double)s_face[fo[a] + c[a] - f[a]];
+                    d[a] = (fx[a] - p[a]) * inv[a];
+                }
+                double ds = d[0];
+                ds = ds > d[1] ? d[1] : ds;
+                ds = ds > d[2] ? d[2] : ds;
+                for (int a = 0; a < 3; ++a) p[a] = (ds == d[a]) ? fx[a] : p[a] + ds * s[a];
+                if (REC) {
+                    const int lin = (c[0] - 1) + n[0] * ((c[1] - 1) + n[1] * (c[2] - 1));
+                    hash = (hash ^ (((uint32_t)lev << 28) ^ (uint32_t)lin)) * 16777619u;
+                }
+                const int idx = s_off[oo[0] + c[0] - 1] + s_off[oo[1] + c[1] - 1] + s_off[oo[2] + c[2] - 1];
+                const double Tj = (double)T[idx];
+                double kap, ibj;
+                if (SPEC == kSpecGreyT4) {
+                    kap = (double)P.kappa_grey;
+                    ibj = Tj * Tj * Tj * Tj;
+                } else {
+                    lookup_f64(P, Tj, bg, band, &kap, &ibj);
+                }
+                ++steps;
+                ++cnt;
+                const double tau1 = tau + kap * ds;
+                if (tau1 >= tau_cut) {
+                    G += tra * ibj;
+                    endt = 0;
+                    done = true;
+                    break;
+                }
+                const double tra1 = exp(-tau1);
+                G += (tra - tra1) * ibj;
+                tau = tau1;
+                tra = tra1;
+                const int ax = (d[0] < d[1]) ? ((d[0] < d[2]) ? 0 : 2) : ((d[1] < d[2]) ? 1 : 2);
+                c[ax] += f[ax] ? -1 : 1;
+                if (c[ax] == 0 || c[ax] == n[ax] + 1) {
+                    const int side = c[ax] != 0;
+                    if ((P.bc_periodic >> (ax * 2 + side)) & 1) {
+                        c[ax] = side ? 1 : n[ax];
+                        p[ax] = (double)s_face[fo[ax] + (side ? 0 : n[ax])];
+                    } else {
+                        c[ax] -= f[ax] ? -1 : 1;
+                        G += tra * (double)s_ibw[(ax * 2 + side) * P.nB + band];
+                        endt = 1;
+                        done = true;
+                        break;
+                    }
+                }
+            }
+        }
+        if (!done) D = 1.0 - tra;
+        pray64[q] = make_double2(w * D, w * G / ii);
+        steps_t += (unsigned long long)steps;
+        if (REC && q < P.nrec) {
+            P.rec_steps[q] = steps;
+            P.rec_end[q] = endt;
+            P.rec_hash[q] = hash;
+            P.rec_D[q] = (float)D;
+            P.rec_G[q] = (float)G;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) steps_t += __shfl_xor(steps_t, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(P.nsteps, steps_t);
+}
+template __global__ void trace_f64<kSpecTable, false>(TraceArgs, double2*);
+template __global__ void trace_f64<kSpecTable, true>(TraceArgs, double2*);
+template __global__ void trace_f64<kSpecGreyT4, false>(TraceArgs, double2*);
+template __global__ void trace_f64<kSpecGreyT4, true>(TraceArgs, double2*);
+
+// Per-slot sums of the FP64 traversal's per-ray (E, A), in ray order, in the
+// configured tally arithmetic (FP64, Kahan-compensated FP32 or plain FP32).
+__global__ void reduce_rays64(const double2* pray, int tally, long long nslots, int R, long long slot0,
+                              const int4* slotinfo, double* E, double* A, double* Q2)
+{
+    long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const long long ts = slotinfo[slot0 + s].x;
+    const double2* p = pray + s * R;
+    if (tally == DNSMC_TALLY_KAHAN_F32) {
+        float e = 0, a = 0, q = 0, ce = 0, ca = 0, cq = 0;
+        for (int r = 0; r < R; ++r) {
+            const float pe = (float)p[r].x, pa = (float)p[r].y;
+            float x = pe - pa;
+            float y = pe - ce, t = e + y; ce = (t - e) - y; e = t;
+            y = pa - ca; t = a + y; ca = (t - a) - y; a = t;
+            y = x * x - cq; t = q + y; cq = (t - q) - y; q = t;
+        }
+        E[ts] += e; A[ts] += a; Q2[ts] += q;
+    } else if (tally == DNSMC_TALLY_F32) {
+        float e = 0, a = 0, q = 0;
+        for (int r = 0; r < R; ++r) {
+            const float pe = (float)p[r].x, pa = (float)p[r].y;
+            float x = pe - pa;
+            e += pe; a += pa; q += x * x;
+        }
+        E[ts] += e; A[ts] += a; Q2[ts] += q;
+    } else {
+        double e = 0, a = 0, q = 0;
+        for (int r = 0; r < R; ++r) {
+            const double x = p[r].x - p[r].y;
+            e += p[r].x; a += p[r].y; q += x * x;
+        }
+        E[ts] += e; A[ts] += a; Q2[ts] += q;
+    }
+}
+
 #define DN_TRACE_INST(S, I, T, R, A) template __global__ void trace<S, I, T, R, A>(TraceArgs);
 #define DN_TRACE_INST2(S, I, T, R) DN_TRACE_INST(S, I, T, R, false) DN_TRACE_INST(S, I, T, R, true)
 #define DN_TRACE_INST3(S, I) DN_TRACE_INST2(S, I, 0, false) DN_TRACE_INST2(S, I, 0, true) DN_TRACE_INST2(S, I, 1, false)

@@ -30,6 +30,10 @@ __global__ void t_range(const float* T, int n0, int n1, int n2, int nbx, int nby
 __global__ void raygen_plain(GenArgs a);
 __global__ void raygen_compat(GenArgs a);
 __global__ void raygen_compat_sorted(GenArgs a);
+template <int SPEC, bool REC>
+__global__ void trace_f64(TraceArgs P, double2* pray64);
+__global__ void reduce_rays64(const double2* pray, int tally, long long nslots, int R, long long slot0,
+                              const int4* slotinfo, double* E, double* A, double* Q2);
 __global__ void finalize(const double* E, const double* A, const double* Q2, long long nemit, int spe, int R,
                          int compat, double* Q, double* Eo, double* Ao, double* sd);
 __global__ void spline_pass(SplineArgs s);
@@ -157,6 +161,7 @@ struct dnsmc_solver {
         DevBuf<uint32_t> d_w0, d_w1;
         DevBuf<float> d_w, d_winv;
         DevBuf<float2> d_pray;   // per-ray (E, A) for the deterministic tallies
+        DevBuf<double2> d_pray64;   // per-ray (E, A) of the FP64 traversal
         DevBuf<float> d_qcdf;    // learned proposal: per-slot CDF rows of the batch
         DevBuf<unsigned> d_claim;   // per-XCD super-chunk counters of the traversal
         hipEvent_t done = nullptr;
@@ -210,7 +215,9 @@ static int check_config(const dnsmc_config& c)
     if (!(c.toll > 0.f && c.toll < 1.f)) return fail(DNSMC_EINVAL, "toll must be in (0, 1)");
     if (c.nsnap < 1) return fail(DNSMC_EINVAL, "nsnap must be >= 1");
     if (c.tally < 0 || c.tally > 3) return fail(DNSMC_EINVAL, "unknown tally mode");
-    if (c.precision != 0) return fail(DNSMC_EINVAL, "precision 1 (FP64 traversal) is not built in this library");
+    if (c.precision < 0 || c.precision > 1) return fail(DNSMC_EINVAL, "precision must be 0 (FP32) or 1 (FP64)");
+    if (c.precision == 1 && c.mode != DNSMC_MODE_PLAIN)
+        return fail(DNSMC_EINVAL, "precision 1 (FP64 traversal) runs the plain estimator only");
     for (int a = 0; a < 3; ++a)
         if (c.n[a] < 1 || c.n[a] > 1023) return fail(DNSMC_EINVAL, "grid sizes must be in [1, 1023]");
     if (c.batch_rays < 1024) return fail(DNSMC_EINVAL, "batch_rays must be >= 1024");
@@ -608,7 +615,8 @@ static int do_run(dnsmc_solver* s)
     const bool det = c.tally != DNSMC_TALLY_F64_ATOMIC;
     // ray batches: up to 2^30 rays (16 GB of descriptors) per launch; the
     // deterministic tallies keep 8 more bytes per ray, so they use 2^27
-    const long long cap = det ? (1ll << 27) : (1ll << 30);
+    const bool f64 = c.precision == 1;
+    const long long cap = f64 ? (1ll << 26) : (det ? (1ll << 27) : (1ll << 30));
     long long slots_per_batch = std::max<long long>(1, std::min<long long>(c.batch_rays, cap) / R);
     if (c.mode == DNSMC_MODE_LEARNED && s->have_mlp)   // CDF rows: at most 8 GB per pipeline
         slots_per_batch = std::max<long long>(1, std::min<long long>(slots_per_batch,
@@ -624,7 +632,8 @@ static int do_run(dnsmc_solver* s)
         HIPCHK(pp.d_w1.ensure(batch_rays));
         HIPCHK(pp.d_w.ensure(batch_rays));
         HIPCHK(pp.d_winv.ensure(batch_rays));
-        if (det) HIPCHK(pp.d_pray.ensure(batch_rays));
+        if (det && !f64) HIPCHK(pp.d_pray.ensure(batch_rays));
+        if (f64) HIPCHK(pp.d_pray64.ensure(batch_rays));
         if (c.mode == DNSMC_MODE_LEARNED && s->have_mlp)
             HIPCHK(pp.d_qcdf.ensure((size_t)std::min(slots_per_batch, nslots) * mlp_noutp(s->mlp_nbin)));
         HIPCHK(pp.d_claim.ensure(256));
@@ -859,10 +868,24 @@ static int do_run(dnsmc_solver* s)
             HIPCHK(hipGetLastError());
             EventPair& et = next_event(s, 1);
             HIPCHK(hipEventRecord(et.a, st));
-            HIPCHK(launch_trace(spec, tab, det, rec, amc, s->ncu, tb_cap, lds_trace, st, P));
+            if (f64) {
+                const size_t lds64 = sizeof(float) * (6 * kLevWords + P.nfaces + P.noffs + P.ncent + 6 * (size_t)d.nB);
+                const int nb = (int)std::min<long long>((nr + 255) / 256, (long long)s->ncu * 16);
+                void (*k)(TraceArgs, double2*) =
+                    spec == kSpecGreyT4 ? (rec ? trace_f64<kSpecGreyT4, true> : trace_f64<kSpecGreyT4, false>)
+                                        : (rec ? trace_f64<kSpecTable, true> : trace_f64<kSpecTable, false>);
+                hipLaunchKernelGGL(k, dim3(nb), dim3(256), lds64, st, P, pp.d_pray64.p);
+            } else {
+                HIPCHK(launch_trace(spec, tab, det, rec, amc, s->ncu, tb_cap, lds_trace, st, P));
+            }
             HIPCHK(hipEventRecord(et.b, st));
             s->st.n_trace_launches++;
-            if (det && !rec) {   // record launches tally with FP64 atomics
+            if (f64) {
+                int nb = (int)((nsl + 255) / 256);
+                hipLaunchKernelGGL(reduce_rays64, dim3(nb), dim3(256), 0, st, (const double2*)pp.d_pray64.p, c.tally,
+                                   nsl, R, b0, (const int4*)s->d_slotinfo.p, P.E, P.A, P.Q2);
+                HIPCHK(hipGetLastError());
+            } else if (det && !rec) {   // record launches tally with FP64 atomics
                 int nb = (int)((nsl + 255) / 256);
                 hipLaunchKernelGGL(reduce_rays, dim3(nb), dim3(256), 0, st, (const float2*)pp.d_pray.p, c.tally, nsl, R,
                                    b0, (const int4*)s->d_slotinfo.p, P.E, P.A, P.Q2);
