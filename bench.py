@@ -120,66 +120,6 @@ def measured_traffic(args, cfg, R):
             "source": os.path.relpath(path, ROOT)}
 
 
-def train_proposal(n_snap, rays, cost0, epochs):
-    """The learned proposal of the channel: trained by the library's MFMA
-    kernels on the plain-run tallies of n_snap seeded 64x96x64 snapshots
-    (parabolic profile 573-955 K + N(0, 10 K)), same table."""
-    import dnsmc_amd
-    from dnsmc_amd import learned, workloads
-    table = workloads.synth_ck_table(11)
-    n1 = (64, 96, 64)
-    t0 = time.time()
-    samples = learned.SampleSet()
-    nbin = None
-    for s in range(n_snap):
-        cfg1 = workloads.channel_config(n1, rays_per_cell=rays, spectral=0, Tw=573.0)
-        S1 = dnsmc_amd.Solver(cfg1)
-        S1.set_tables(workloads.table_arrays(table))
-        S1.set_record(True)
-        S1.set_field(0, workloads.ghosted(workloads.field_profile_noise(31 + s, n1)))
-        S1.run()
-        S1.wait()
-        samples.add_run(S1, cost0=cost0)
-        sc = S1.array("scalars")
-        nbin = int(sc[3]) * int(sc[4])
-        S1.close()
-    t1 = time.time()
-    data = samples.cat()
-    net = learned.ProposalNet(nbin, seed=0)
-    hist = learned.train(net, data, epochs=epochs, batch=16384, lr=2e-3, seed=0)
-    t2 = time.time()
-    log("proposal: %d samples from %d snapshots in %.1f s, %d epochs in %.1f s, loss %s"
-        % (data[0].shape[0], n_snap, t1 - t0, epochs, t2 - t1, ["%.4f" % h for h in hist]))
-    info = dict(train_snapshots=n_snap, rays_per_cell=rays, samples=int(data[0].shape[0]), epochs=epochs,
-                cost0=cost0, collect_s=round(t1 - t0, 2), train_s=round(t2 - t1, 2),
-                train_samples_per_s=data[0].shape[0] * epochs / max(t2 - t1, 1e-9), loss=[round(h, 5) for h in hist])
-    return net.host_weights(), info
-
-
-def oracle_reference(S, cfg, fields, cells, R, R_ref, seed):
-    """Converged FP64 reference MC (the oracle, FP64 rays and tallies) on a
-    subset of emitting cells of snapshot 0, scaled to the solver's bundle
-    energy (its weights are built for R rays per cell)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    o = pyoracle.get()
-    P = o.problem(oracle_problem(S, cfg, fields[0]))
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    t0 = time.time()
-    E, A, Q2, ns = o.mc_philox(P, seed, 0, R_ref, cells=np.ascontiguousarray(cells, np.int32),
-                               nthreads=max(1, min(threads, 64)), f64=True)
-    dt = time.time() - t0
-    s = R / R_ref
-    Q = (E - A) * s
-    var = (R_ref * Q2 - (E - A) ** 2) / (R_ref - 1) * s * s
-    return Q, var, dt
-
-
-def rms_error(Q, Qref, var_ref):
-    num = max(float(np.sum((Q - Qref) ** 2) - np.sum(var_ref)), 0.0)
-    return math.sqrt(num / float(np.sum(Qref ** 2)))
-
-
 def timed_solve(S, steps):
     import dnsmc_amd
     S.run()
@@ -212,8 +152,8 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--cost0", type=float, default=16.0)
     ap.add_argument("--mix-eps", type=float, default=0.05)
-    ap.add_argument("--ref-cells", type=int, default=768)
-    ap.add_argument("--ref-rays", type=int, default=16384)
+    ap.add_argument("--ref-cells", type=int, default=1024)
+    ap.add_argument("--ref-rays", type=int, default=65536)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -305,20 +245,22 @@ def main():
     if not args.no_learned:
         # time to 1% RMS error of Q_rad against a converged FP64 reference,
         # plain vs learned proposal, at the same rays per cell
+        from dnsmc_amd import learned, study
         res_p = S.results(0)
-        ncell = int(np.prod(cfg.n))
-        rng = np.random.Generator(np.random.Philox(key=11))
-        cells = np.sort(rng.choice(ncell, size=min(args.ref_cells, ncell), replace=False))
-        Qref, vref, t_ref = oracle_reference(S, cfg, fields, cells, R, args.ref_rays, seed=987654321)
         S.close()
-        weights, info = train_proposal(args.train_snaps, args.train_rays, args.cost0, args.train_epochs)
+        cells = study.sample_cells(args.workload, args.ref_cells)
+        Qref, vref, t_ref = study.reference(args.workload, cells, args.ref_rays, fields=fields)
+        weights, info = learned.train_channel_proposal(args.train_snaps, args.train_rays, args.cost0,
+                                                       args.train_epochs)
+        log("proposal: %s" % (info,))
         SL, cfgl, _, _, _ = setup(args.workload, args.rays, args.nsnap, mode=2, batch=args.batch, mlp=weights,
                                   mix_eps=args.mix_eps)
         dt_l, st_l = timed_solve(SL, 1)
         res_l = SL.results(0)
         SL.close()
-        eps_p = rms_error(res_p["Q"][cells], Qref, vref)
-        eps_l = rms_error(res_l["Q"][cells], Qref, vref)
+        err_p = study.errors(res_p["Q"][cells], Qref, vref)
+        err_l = study.errors(res_l["Q"][cells], Qref, vref)
+        eps_p, eps_l = err_p["rms"], err_l["rms"]
         t_p = ms_step / 1000.0 * (eps_p / 0.01) ** 2
         t_l = dt_l * (eps_l / 0.01) ** 2
         var_ratio = float(np.sum(res_p["sdev"] ** 2) / np.sum(res_l["sdev"] ** 2))
@@ -327,8 +269,10 @@ def main():
                               segments_per_ray=st_l[0]["n_steps"] / max(st_l[0]["n_rays"], 1),
                               variance_ratio_vs_plain=var_ratio)
         out["rms_error_vs_fp64_oracle"] = dict(
-            plain=eps_p, learned=eps_l, rays_per_cell=R, snapshot=0, cells=int(len(cells)),
-            reference=dict(rays_per_cell=args.ref_rays, kind="oracle FP64 rays and tallies", seconds=round(t_ref, 1)),
+            plain=eps_p, learned=eps_l, max_plain=err_p["max"], max_learned=err_l["max"], rays_per_cell=R,
+            snapshot=0, cells=int(len(cells)),
+            reference=dict(rays_per_cell=args.ref_rays, seconds=round(t_ref, 1),
+                           kind="converged FP64 MC (FP64 traversal and tallies, other seed), the oracle's estimator"),
             estimate_from_variance=dict(plain=math.sqrt(np.sum(res_p["sdev"] ** 2) / np.sum(res_p["Q"] ** 2)),
                                         learned=math.sqrt(np.sum(res_l["sdev"] ** 2) / np.sum(res_l["Q"] ** 2))))
         out["time_to_1pct_rms_s"] = dict(plain=t_p, learned=t_l, speedup_learned=t_p / t_l,

@@ -152,3 +152,39 @@ def train(net, samples, epochs=3, batch=8192, lr=2e-3, seed=0, log=None):
         if log:
             log("epoch %d loss %.5f" % (ep, hist[-1]))
     return hist
+
+
+def train_channel_proposal(n_snap=20, rays=8, cost0=16.0, epochs=2, batch=16384, lr=2e-3, log=None):
+    """The channel's proposal: plain-run tallies of `n_snap` seeded 64x96x64
+    snapshots (parabolic profile 573-955 K + N(0, 10 K), seeds 31..) with the
+    synthetic 100x16x69 table, `rays` rays per cell, then `epochs` of Adam.
+    Returns (weights for Solver.set_mlp, info dict)."""
+    import time
+    from . import workloads
+    from .solver import Solver
+    table = workloads.synth_ck_table(11)
+    n1 = (64, 96, 64)
+    t0 = time.time()
+    samples = SampleSet()
+    nbin = None
+    for s in range(n_snap):
+        S = Solver(workloads.channel_config(n1, rays_per_cell=rays, spectral=0, Tw=573.0))
+        S.set_tables(workloads.table_arrays(table))
+        S.set_record(True)
+        S.set_field(0, workloads.ghosted(workloads.field_profile_noise(31 + s, n1)))
+        S.run()
+        S.wait()
+        samples.add_run(S, cost0=cost0)
+        sc = S.array("scalars")
+        nbin = int(sc[3]) * int(sc[4])
+        S.close()
+    t1 = time.time()
+    data = samples.cat()
+    net = ProposalNet(nbin, seed=0)
+    hist = train(net, data, epochs=epochs, batch=batch, lr=lr, seed=0, log=log)
+    t2 = time.time()
+    nsamp = int(data[0].shape[0])
+    info = dict(train_snapshots=n_snap, rays_per_cell=rays, samples=nsamp, epochs=epochs, cost0=cost0,
+                collect_s=round(t1 - t0, 2), train_s=round(t2 - t1, 2),
+                train_samples_per_s=nsamp * epochs / max(t2 - t1, 1e-9), loss=[round(h, 5) for h in hist])
+    return net.host_weights(), info

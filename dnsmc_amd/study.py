@@ -1,0 +1,200 @@
+"""Accuracy studies of the radiation solve on the channel workloads.
+
+Reference values come from the library itself: the FP64 traversal
+(precision 1: FP64 directions, lookups, transport, weights and tallies) run
+with many rays on a seeded subset of emitting cells.  It is the same
+estimator as the CPU FP64 oracle, to 1e-12 on the same rays (the GPU parity
+tests check exactly that), on the GPU's throughput.
+
+  python -m dnsmc_amd.study errors       RMS / max error of Q_rad per workload
+  python -m dnsmc_amd.study precision    FP32 vs FP64 traversal x FP64 vs
+                                         Kahan-FP32 tallies, 256x128x128
+  python -m dnsmc_amd.study convergence  error vs rays per cell (R^-1/2)
+
+Errors over the chosen cells: eps_rms = sqrt(sum (Q - Q_ref)^2 - sum var_ref)
+/ sqrt(sum Q_ref^2) (the reference's own variance removed), eps_max =
+max |Q - Q_ref| / max |Q_ref|.
+"""
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import workloads
+from .solver import Solver
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_solver(workload, rays, nsnap=1, mode=0, precision=0, tally=0, seed=None, cells=None, mlp=None,
+                mix_eps=0.1, batch=None, fields=None):
+    """Solver of a named channel workload with its table and seeded snapshots."""
+    w = workloads.WORKLOADS[workload]
+    kw = dict(mode=mode, precision=precision, tally=tally)
+    if seed is not None:
+        kw["seed"] = seed
+    if batch:
+        kw["batch_rays"] = int(batch)
+    cfg = workloads.channel_config(w["n"], rays_per_cell=rays, spectral=w["spectral"], Tw=w["Tw"], nsnap=nsnap,
+                                   **kw)
+    S = Solver(cfg)
+    if w["spectral"] == 3:
+        S.load_tables()
+    else:
+        S.set_tables(workloads.table_arrays(workloads.synth_ck_table(11)))
+    if mlp is not None:
+        S.set_mlp(*mlp, mix_eps=mix_eps)
+    if cells is not None:
+        S.set_emitters(cells)
+    if fields is None:
+        fields = workloads.workload_fields(workload, nsnap)
+    for s in range(nsnap):
+        S.set_field(s, workloads.ghosted(fields[s]))
+    return S, cfg
+
+
+def sample_cells(workload, k, key=11):
+    n = workloads.WORKLOADS[workload]["n"]
+    ncell = n[0] * n[1] * n[2]
+    rng = np.random.Generator(np.random.Philox(key=key))
+    return np.sort(rng.choice(ncell, size=min(k, ncell), replace=False)).astype(np.int32)
+
+
+def reference(workload, cells, rays_ref, seed=987654321, fields=None):
+    """Converged FP64 reference MC of snapshot 0 on `cells`: (Q, var(Q), seconds)."""
+    t0 = time.time()
+    S, _ = make_solver(workload, rays_ref, precision=1, tally=1, seed=seed, cells=cells,
+                       fields=None if fields is None else fields[:1])
+    S.run()
+    r = S.results(0)
+    S.close()
+    return r["Q"], r["sdev"] ** 2, time.time() - t0
+
+
+def errors(Q, Qref, var_ref=None):
+    d2 = float(np.sum((Q - Qref) ** 2))
+    if var_ref is not None:
+        d2 = max(d2 - float(np.sum(var_ref)), 0.0)
+    return dict(rms=math.sqrt(d2 / float(np.sum(Qref ** 2))),
+                max=float(np.max(np.abs(Q - Qref)) / np.max(np.abs(Qref))))
+
+
+def study_errors(workloads_=("grey32", "ck64", "ck256"), ncells=1024, rays_ref=65536):
+    out = {}
+    for wl in workloads_:
+        w = workloads.WORKLOADS[wl]
+        cells = sample_cells(wl, ncells)
+        Qref, vref, tref = reference(wl, cells, rays_ref)
+        S, cfg = make_solver(wl, w["rays"])
+        t0 = time.time()
+        S.run()
+        st = S.stats()
+        dt = time.time() - t0
+        r = S.results(0)
+        S.close()
+        e = errors(r["Q"][cells], Qref, vref)
+        e_est = math.sqrt(float(np.sum(r["sdev"][cells] ** 2)) / float(np.sum(Qref ** 2)))
+        out[wl] = dict(grid=list(cfg.n), rays_per_cell=w["rays"], rms=e["rms"], max=e["max"],
+                       rms_from_own_variance=e_est, solve_s=dt, rays_per_s=st["n_rays"] / dt, cells=len(cells),
+                       reference=dict(rays_per_cell=rays_ref, seconds=round(tref, 2)))
+        log(wl, out[wl])
+    return out
+
+
+def study_precision(ncells=1024, rays_ref=65536):
+    """256x128x128 (profile + Fourier modes), 200 rays/cell, the four
+    arithmetic variants on the same Philox rays.  'same_rays' compares with
+    the FP64 traversal + FP64 tallies on the same rays (the arithmetic error
+    alone); 'converged' with the converged FP64 reference (arithmetic plus
+    Monte Carlo error)."""
+    wl = "ck256"
+    R = workloads.WORKLOADS[wl]["rays"]
+    cells = sample_cells(wl, ncells)
+    fields = workloads.workload_fields(wl, 1)
+    Qref, vref, tref = reference(wl, cells, rays_ref, fields=fields)
+    runs = {}
+    for prec in (0, 1):
+        for tally, tname in ((1, "fp64"), (2, "kahan_fp32")):
+            S, _ = make_solver(wl, R, precision=prec, tally=tally, fields=fields)
+            t0 = time.time()
+            S.run()
+            S.wait()
+            dt = time.time() - t0
+            r = S.results(0)
+            S.close()
+            runs[("fp32" if prec == 0 else "fp64", tname)] = (r, dt)
+    base = runs[("fp64", "fp64")][0]
+    out = {}
+    for (trav, tname), (r, dt) in runs.items():
+        Qs = base["Q"]
+        same = errors(r["Q"], Qs)
+        same_E = float(np.max(np.abs(r["E"] - base["E"]) / np.abs(base["E"])))
+        conv = errors(r["Q"][cells], Qref, vref)
+        out["%s_traversal_%s_tally" % (trav, tname)] = dict(
+            same_rays_rms=same["rms"], same_rays_max=same["max"], same_rays_max_rel_E=same_E,
+            converged_rms=conv["rms"], converged_max=conv["max"], solve_s=round(dt, 3))
+        log(trav, tname, out["%s_traversal_%s_tally" % (trav, tname)])
+    return dict(workload=wl, rays_per_cell=R, cells_for_converged=len(cells),
+                reference=dict(rays_per_cell=rays_ref, seconds=round(tref, 2)), variants=out)
+
+
+def study_convergence(workload="ck384x8", rays=(100, 200, 400, 800), ncells=1024, rays_ref=65536, mlp=None,
+                      mix_eps=0.05):
+    """Error against the converged reference as rays per cell double
+    (snapshot 0 only); the fitted exponent of eps ~ R^p should be -1/2."""
+    cells = sample_cells(workload, ncells)
+    fields = workloads.workload_fields(workload, 1)
+    Qref, vref, _ = reference(workload, cells, rays_ref, fields=fields)
+    res = {}
+    legs = [("plain", 0, None)] + ([("learned", 2, mlp)] if mlp is not None else [])
+    for name, mode, W in legs:
+        pts = []
+        for R in rays:
+            S, _ = make_solver(workload, R, mode=mode, mlp=W, mix_eps=mix_eps, fields=fields)
+            S.run()
+            S.wait()
+            t0 = time.time()
+            S.run()
+            S.wait()
+            dt = time.time() - t0
+            r = S.results(0)
+            S.close()
+            e = errors(r["Q"][cells], Qref, vref)
+            pts.append(dict(rays_per_cell=R, rms=e["rms"], max=e["max"], seconds=dt))
+            log(name, pts[-1])
+        lr = np.log([p["rays_per_cell"] for p in pts])
+        le = np.log([p["rms"] for p in pts])
+        slope = float(np.polyfit(lr, le, 1)[0])
+        # seconds to 1% RMS from the fitted power law (time proportional to rays)
+        c = np.polyfit(lr, le, 1)
+        R1 = math.exp((math.log(0.01) - c[1]) / c[0])
+        sec_per_ray = np.mean([p["seconds"] / p["rays_per_cell"] for p in pts])
+        res[name] = dict(points=pts, exponent=slope, rays_per_cell_to_1pct=R1, seconds_to_1pct=R1 * sec_per_ray)
+    if "learned" in res:
+        res["speedup_learned"] = res["plain"]["seconds_to_1pct"] / res["learned"]["seconds_to_1pct"]
+    return dict(workload=workload, snapshot=0, cells=len(cells), reference_rays_per_cell=rays_ref, legs=res)
+
+
+def main(argv):
+    what = argv[1] if len(argv) > 1 else "errors"
+    if what == "errors":
+        out = study_errors()
+    elif what == "precision":
+        out = study_precision()
+    elif what == "convergence":
+        mlp = None
+        if "--learned" in argv:
+            from .learned import train_channel_proposal
+            mlp, _ = train_channel_proposal()
+        out = study_convergence(mlp=mlp)
+    else:
+        raise SystemExit("usage: python -m dnsmc_amd.study errors|precision|convergence [--learned]")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv)
