@@ -365,8 +365,37 @@ __device__ __forceinline__ float load_T(gfloat* base, int idx)
     return *(gfloat*)((gchar*)base + (uint32_t)idx * 4u);
 }
 
-// Tally one finished ray into the lane's running slot sums (TMODE 0) or the
-// per-ray buffer (TMODE 1).
+// Wave-aggregated tally atomics: the lanes with `need` set hand their running
+// slot sums over, grouped by slot; each group is summed across the wave and
+// its first lane adds the group total with three FP64 atomics.  Called by all
+// 64 lanes (wave-uniform control flow).
+__device__ __forceinline__ void flush_wave(const TraceArgs& P, bool need, int aslot, double aE, double aA,
+                                           double aQ, int lane)
+{
+    unsigned long long m = __ballot(need);
+    while (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        const int s = __shfl(aslot, leader);
+        const bool mine = need && aslot == s;
+        double e = mine ? aE : 0.0, a = mine ? aA : 0.0, q = mine ? aQ : 0.0;
+        for (int off = 32; off > 0; off >>= 1) {
+            e += __shfl_xor(e, off);
+            a += __shfl_xor(a, off);
+            q += __shfl_xor(q, off);
+        }
+        if (lane == leader) {
+            unsafeAtomicAdd(&P.E[s], e);
+            unsafeAtomicAdd(&P.A[s], a);
+            unsafeAtomicAdd(&P.Q2[s], q);
+        }
+        need = need && !mine;
+        m = __ballot(need);
+    }
+}
+
+// Tally one finished ray into the lane's running slot sums (TMODE 0; a lane
+// whose sums belong to another slot has handed them over with flush_wave
+// first) or the per-ray buffer (TMODE 1).
 template <int TMODE>
 __device__ __forceinline__ void tally_ray(const TraceArgs& P, float er, float ar, int rslot, uint32_t q, int& aslot,
                                           double& aE, double& aA, double& aQ)
@@ -376,11 +405,6 @@ __device__ __forceinline__ void tally_ray(const TraceArgs& P, float er, float ar
         return;
     }
     if (rslot != aslot) {
-        if (aslot >= 0) {
-            unsafeAtomicAdd(&P.E[aslot], aE);
-            unsafeAtomicAdd(&P.A[aslot], aA);
-            unsafeAtomicAdd(&P.Q2[aslot], aQ);
-        }
         aslot = rslot;
         aE = aA = aQ = 0.0;
     }
@@ -507,6 +531,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         if (na == 0 && !have_rays) break;
         if (have_rays && (ni >= P.refill_k || na == 0)) {
             // ---- tally the rays finished since the last refill
+            if (TMODE == 0) flush_wave(P, fin && rslot != aslot && aslot >= 0, aslot, aE, aA, aQ, lane);
             if (fin) {
                 tally_ray<TMODE>(P, rw * Dfin, rwi * G, rslot, qcur, aslot, aE, aA, aQ);
                 if (REC && qcur < P.nrec) {
@@ -728,6 +753,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         }
     }
     // a ray that ended after the last refill (nothing was left to hand out)
+    if (TMODE == 0) flush_wave(P, fin && rslot != aslot && aslot >= 0, aslot, aE, aA, aQ, lane);
     if (fin) {
         tally_ray<TMODE>(P, rw * Dfin, rwi * G, rslot, qcur, aslot, aE, aA, aQ);
         if (REC && qcur < P.nrec) {
@@ -739,11 +765,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         }
         steps_lane += (uint32_t)rsteps;
     }
-    if (TMODE == 0 && aslot >= 0) {
-        unsafeAtomicAdd(&P.E[aslot], aE);
-        unsafeAtomicAdd(&P.A[aslot], aA);
-        unsafeAtomicAdd(&P.Q2[aslot], aQ);
-    }
+    if (TMODE == 0) flush_wave(P, aslot >= 0, aslot, aE, aA, aQ, lane);
     // one step-count atomic per wave
     unsigned long long st = steps_lane;
     for (int off = 32; off > 0; off >>= 1) st += __shfl_xor(st, off);
