@@ -81,6 +81,18 @@ struct TraceArgs {
     double* A;
     double* Q2;
     unsigned long long* nsteps;
+    // several snapshots in one launch (fused generation): rays of snapshot
+    // P.snap + s are [0, nrays) again; tally slot = slotinfo.x + s * tally_stride
+    int nsnap_launch;
+    int tally_stride;
+    // fused Philox generation (plain estimator, code/mc_gpu.cu:93-144 semantics
+    // of the plain draw): counter (r, cell, snapshot, 0), key seed
+    unsigned long long seed;
+    const float* cdf_band;
+    const float* cdf_g;
+    const float* inv_emax;
+    float Ibmax;
+    int g_search;
     // FP64 traversal extras
     const double* inv_emax_d;
     double Ibmax_d;

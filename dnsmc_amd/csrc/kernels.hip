@@ -441,7 +441,13 @@ __device__ __forceinline__ void tally_ray(const TraceArgs& P, float er, float ar
 // geometry (distances, face snap, move, boundary) runs while it is in flight,
 // the next cell's temperature load is issued at the end of the geometry, and
 // the optical depth / transmissivity / tally update consumes kappa last.
-template <int SPEC, int TAB, int TMODE, bool REC, bool AMC>
+//
+// GEN: the rays are drawn in the refill itself (the plain estimator's Philox
+// draw, bit-identical to raygen_plain) instead of read from descriptors, and
+// one launch covers every snapshot: XCD x's claims walk snapshot by snapshot
+// through its 1/8 of each snapshot's super-chunks, so all XCDs work on the
+// same snapshot at a time.
+template <int SPEC, int TAB, int TMODE, bool REC, bool AMC, bool GEN>
 __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArgs P)
 {
     extern __shared__ float lds[];
@@ -453,6 +459,8 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     float* s_ib = s_ibw + 6 * P.nB;             // Ib rows (nT or tab_nw nodes)
     const int nw = P.tab_nw;
     float* s_kq = s_ib + P.nB * nw;             // sliced kappa rows (kTabSliced)
+    // GEN: band CDF after the tables that are in LDS
+    float* s_cdfb = TAB == kTabSliced ? s_kq + P.nB * P.nQ * nw : (TAB == kTabIbLds ? s_ib + P.nB * P.nT : s_ib);
     for (int i = threadIdx.x; i < P.nlev; i += blockDim.x) {
         int* L = s_lev + i * kLevWords;
         L[0] = P.lev[i].n[0]; L[1] = P.lev[i].n[1]; L[2] = P.lev[i].n[2];
@@ -467,11 +475,14 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         L[11] = P.lev[i].ooff[0];
         L[12] = P.lev[i].ooff[1];
         L[13] = P.lev[i].ooff[2];
+        L[14] = (int)P.lev[i].tstride;
     }
     for (int i = threadIdx.x; i < P.nfaces; i += blockDim.x) s_face[i] = P.faces[i];
     for (int i = threadIdx.x; i < P.noffs; i += blockDim.x) s_off[i] = P.offs[i];
     for (int i = threadIdx.x; i < P.ncent; i += blockDim.x) s_cent[i] = P.centres[i];
     for (int i = threadIdx.x; i < 6 * P.nB; i += blockDim.x) s_ibw[i] = P.Ibw[i];
+    if (GEN)
+        for (int i = threadIdx.x; i < P.nB; i += blockDim.x) s_cdfb[i] = P.cdf_band[i];
     if (SPEC == kSpecTable && TAB == kTabIbLds)
         for (int i = threadIdx.x; i < P.nB * P.nT; i += blockDim.x) s_ib[i] = P.Ib[i];
     if (SPEC == kSpecTable && TAB == kTabSliced) {
@@ -491,18 +502,19 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
 
     const int lane = threadIdx.x & 63;
     const unsigned nb = gridDim.x, b = blockIdx.x;
-    const uint32_t nrays = (uint32_t)P.nrays;
+    const uint32_t nrays = (uint32_t)P.nrays;   // rays per snapshot of the launch
     const uint32_t S = (uint32_t)P.super;
-    const uint32_t nsc_all = (nrays + S - 1) / S;
+    const uint32_t cps = (nrays + S - 1) / S;   // super-chunks per snapshot
     const bool split = (nb % 8u) == 0u;
     const uint32_t xcd = split ? b % 8u : 0u;
     const uint32_t nparts = split ? 8u : 1u;
-    const uint32_t per_xcd = (nsc_all + nparts - 1u) / nparts;
+    const uint32_t per_x = (cps + nparts - 1u) / nparts;        // per XCD and snapshot
+    const uint32_t nclaim = per_x * (uint32_t)P.nsnap_launch;   // claims of one XCD counter
     const int nT = P.nT;
     const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
 
-    // wave-uniform cursor over the claimed super-chunk
-    uint32_t xi = 0, cur = 0, cend = 0;
+    // wave-uniform cursor over the claimed super-chunk (rays cur..cend of snapshot csnap)
+    uint32_t xi = 0, cur = 0, cend = 0, csnap = 0;
     bool more = true;
     // lane state
     bool alive = false, fin = false;
@@ -521,6 +533,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     int oo0 = s_lev[11], oo1 = s_lev[12], oo2 = s_lev[13];
     gfloat* Tl = lev_T(s_lev);
     gfloat* T0 = Tl;
+    uint32_t rsnap = 0;
 
     for (;;) {
         const bool idle = !alive;
@@ -547,32 +560,41 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
             // ---- hand out consecutive rays to the idle lanes, in lane order
             const uint32_t rank = (uint32_t)__popcll(m_idle & lanemask_lt);
             const uint32_t avail = cend - cur;
-            uint32_t q = 0xFFFFFFFFu;
+            uint32_t q = 0xFFFFFFFFu, qs = csnap;
             if (idle && rank < avail) q = cur + rank;
             if (avail < (uint32_t)ni && more) {
                 // the current super-chunk runs out: claim the next one, from this
-                // XCD's range first, then from the following XCDs' ranges
-                uint32_t sc = 0xFFFFFFFFu;
+                // XCD's share first, then from the following XCDs' shares.  Claim
+                // c of XCD X is chunk X*per_x + c % per_x of snapshot c / per_x.
+                bool got = false;
+                uint32_t nstart = 0, nend = 0, nsn = 0;
                 while (xi < nparts) {
                     const uint32_t X = (xcd + xi) % nparts;
-                    const uint32_t base = X * per_xcd;
-                    const uint32_t cnt = base >= nsc_all ? 0u : (nsc_all - base < per_xcd ? nsc_all - base : per_xcd);
                     uint32_t c = 0;
                     if (lane == 0) c = atomicAdd(P.claim + 32u * X, 1u);
                     c = __builtin_amdgcn_readfirstlane(c);
-                    if (c < cnt) {
-                        sc = base + c;
-                        break;
+                    if (c >= nclaim) {
+                        ++xi;
+                        continue;
                     }
-                    ++xi;
+                    const uint32_t sn = c / per_x;
+                    const uint32_t lc = X * per_x + (c - sn * per_x);
+                    if (lc >= cps) continue;   // past the snapshot's last chunk
+                    nsn = sn;
+                    nstart = lc * S;
+                    nend = nstart + S < nrays ? nstart + S : nrays;
+                    got = true;
+                    break;
                 }
-                if (sc != 0xFFFFFFFFu) {
-                    const uint32_t nstart = sc * S;
-                    const uint32_t nend = nstart + S < nrays ? nstart + S : nrays;
-                    if (idle && rank >= avail && nstart + (rank - avail) < nend) q = nstart + (rank - avail);
+                if (got) {
+                    if (idle && rank >= avail && nstart + (rank - avail) < nend) {
+                        q = nstart + (rank - avail);
+                        qs = nsn;
+                    }
                     const uint32_t used = (uint32_t)ni - avail;
                     cur = nstart + used < nend ? nstart + used : nend;
                     cend = nend;
+                    csnap = nsn;
                 } else {
                     more = false;
                     cur = cend;
@@ -583,15 +605,56 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
             // ---- start the handed-out rays
             if (q != 0xFFFFFFFFu) {
                 qcur = q;
-                const uint32_t a0 = P.w0[q], a1 = P.w1[q];
-                rw = P.w[q];
-                rwi = P.winv[q];
-                const int4 si = P.slotinfo[P.slot0 + fdiv(q, P.fd_R)];
-                rslot = si.x;
-                band = desc_band(a0, a1);
-                const int g = desc_g(a1);
-                const float ut = ((float)(a0 & 0xFFFFFFu) + P.u_off) * kTwoM24;
-                const float up = ((float)(a1 & 0xFFFFFFu) + P.u_off) * kTwoM24;
+                rsnap = qs;
+                const uint32_t slot = fdiv(q, P.fd_R);
+                const int4 si = P.slotinfo[P.slot0 + slot];
+                rslot = si.x + (int)rsnap * P.tally_stride;
+                if (GEN) T0 = lev_T(s_lev) + (size_t)rsnap * (uint32_t)s_lev[14];
+                Tc = T0[si.z];
+                int g;
+                uint32_t mt, mp;
+                if (GEN) {
+                    // the plain draw of ray r of the cell (raygen_plain)
+                    const uint32_t r = q - slot * (uint32_t)P.rays_per_slot;
+                    const u32x4 x = philox4x32_10(u32x4{r, (uint32_t)si.y, (uint32_t)(P.snap + (int)rsnap), 0u},
+                                                  (uint32_t)P.seed, (uint32_t)(P.seed >> 32));
+                    band = cdf_search(s_cdfb, P.nB, u01_open(x.x));
+                    g = P.g_search ? cdf_search(P.cdf_g + band * P.nQ, P.nQ, u01_open(x.y)) : 0;
+                    g = g > P.nQ - 1 ? P.nQ - 1 : g;
+                    float kap, ib;
+                    if (SPEC == kSpecGreyT4) {
+                        const float t2 = Tc * Tc;
+                        kap = P.kappa_grey;
+                        ib = t2 * t2;
+                    } else {
+                        float pos = fmaf(Tc, P.inv_dT, P.mT0);
+                        pos = fminf(fmaxf(pos, 0.0f), (float)(nT - 1));
+                        int t0 = (int)pos;
+                        t0 = t0 > nT - 2 ? nT - 2 : t0;
+                        const float fr = pos - (float)t0;
+                        const int kr = (band * P.nQ + g) * rowlen, ir = band * rowlen;
+                        const int t = TAB == kTabSliced ? max(0, min(t0 - P.tab_lo, nw - 2)) : t0;
+                        const float k0 = TAB == kTabSliced ? s_kq[kr + t] : kq[kr + t];
+                        const float k1 = TAB == kTabSliced ? s_kq[kr + t + 1] : kq[kr + t + 1];
+                        const float i0 = ibt[ir + t], i1 = ibt[ir + t + 1];
+                        kap = fmaf(fr, k1 - k0, k0);
+                        ib = fmaf(fr, i1 - i0, i0);
+                    }
+                    rw = P.Ibmax * (kap * ib) * P.inv_emax[band * P.nQ + g];
+                    rwi = rw * (1.0f / ib);
+                    mt = x.z >> 8;
+                    mp = x.w >> 8;
+                } else {
+                    const uint32_t a0 = P.w0[q], a1 = P.w1[q];
+                    rw = P.w[q];
+                    rwi = P.winv[q];
+                    band = desc_band(a0, a1);
+                    g = desc_g(a1);
+                    mt = a0 & 0xFFFFFFu;
+                    mp = a1 & 0xFFFFFFu;
+                }
+                const float ut = ((float)mt + P.u_off) * kTwoM24;
+                const float up = ((float)mp + P.u_off) * kTwoM24;
                 const float cth = 1.0f - 2.0f * ut;
                 const float sth = 2.0f * sqrtf(ut * (1.0f - ut));
                 float sp, cp;
@@ -618,13 +681,12 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                     n0 = s_lev[0]; n1 = s_lev[1]; n2 = s_lev[2];
                     fo0 = s_lev[3]; fo1 = s_lev[4]; fo2 = s_lev[5]; maxs = s_lev[6];
                     oo0 = s_lev[11]; oo1 = s_lev[12]; oo2 = s_lev[13];
-                    Tl = T0;
                 }
+                Tl = T0;
                 fx = s_face[fo0 + cx - (fl & 1)];
                 fy = s_face[fo1 + cy - ((fl >> 1) & 1)];
                 fz = s_face[fo2 + cz - ((fl >> 2) & 1)];
                 idx = si.z;
-                Tc = T0[idx];
                 tau = 0.0f;
                 tra = 1.0f;
                 G = 0.0f;
@@ -736,6 +798,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                     fo0 = L[3]; fo1 = L[4]; fo2 = L[5]; maxs = L[6];
                     oo0 = L[11]; oo1 = L[12]; oo2 = L[13];
                     Tl = lev_T(L);
+                    if (GEN) Tl += (size_t)rsnap * (uint32_t)L[14];
                     const int q0 = o0 / n0, q1 = o1 / n1, q2 = o2 / n2;
                     cx = (cx + q0 - 1) / q0;
                     cy = (cy + q1 - 1) / q1;
@@ -983,9 +1046,10 @@ __global__ void reduce_rays64(const double2* pray, int tally, long long nslots, 
     }
 }
 
-#define DN_TRACE_INST(S, I, T, R, A) template __global__ void trace<S, I, T, R, A>(TraceArgs);
-#define DN_TRACE_INST2(S, I, T, R) DN_TRACE_INST(S, I, T, R, false) DN_TRACE_INST(S, I, T, R, true)
-#define DN_TRACE_INST3(S, I) DN_TRACE_INST2(S, I, 0, false) DN_TRACE_INST2(S, I, 0, true) DN_TRACE_INST2(S, I, 1, false)
+#define DN_TRACE_INST(S, I, T, R, A, G) template __global__ void trace<S, I, T, R, A, G>(TraceArgs);
+#define DN_TRACE_INST2(S, I, T, R, G) DN_TRACE_INST(S, I, T, R, false, G) DN_TRACE_INST(S, I, T, R, true, G)
+#define DN_TRACE_INST3(S, I) DN_TRACE_INST2(S, I, 0, false, false) DN_TRACE_INST2(S, I, 0, true, false) \
+    DN_TRACE_INST2(S, I, 1, false, false) DN_TRACE_INST2(S, I, 0, false, true)
 DN_TRACE_INST3(kSpecTable, kTabSliced)
 DN_TRACE_INST3(kSpecTable, kTabIbLds)
 DN_TRACE_INST3(kSpecTable, kTabGlobal)

@@ -24,7 +24,7 @@
 
 namespace dn {
 // kernels.hip
-template <int SPEC, int TAB, int TMODE, bool REC, bool AMC>
+template <int SPEC, int TAB, int TMODE, bool REC, bool AMC, bool GEN>
 __global__ void trace(TraceArgs P);
 __global__ void t_range(const float* T, int n0, int n1, int n2, int nbx, int nby, unsigned* lohi);
 __global__ void raygen_plain(GenArgs a);
@@ -515,10 +515,10 @@ static void build_slotinfo(const std::vector<int>& emit, const std::vector<int>&
 // device for this kernel variant (blocks per CU from the occupancy query,
 // times the CU count), capped by `max_blocks`, a multiple of 8 (one share per
 // XCD).  The sliced-table variant runs one 1024-thread block per CU.
-template <int SPEC, int TAB, int TM, bool REC>
+template <int SPEC, int TAB, int TM, bool REC, bool GEN = false>
 static hipError_t launch_trace_t(bool amc, int ncu, int max_blocks, size_t lds, hipStream_t st, const TraceArgs& P)
 {
-    void (*k)(TraceArgs) = amc ? trace<SPEC, TAB, TM, REC, true> : trace<SPEC, TAB, TM, REC, false>;
+    void (*k)(TraceArgs) = amc ? trace<SPEC, TAB, TM, REC, true, GEN> : trace<SPEC, TAB, TM, REC, false, GEN>;
     const int threads = TAB == kTabSliced ? 1024 : 256;
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, lds);
@@ -532,21 +532,23 @@ static hipError_t launch_trace_t(bool amc, int ncu, int max_blocks, size_t lds, 
 }
 
 template <int SPEC, int TAB>
-static hipError_t launch_trace_m(bool det, bool rec, bool amc, int ncu, int max_blocks, size_t lds, hipStream_t st,
-                                 const TraceArgs& P)
+static hipError_t launch_trace_m(bool gen, bool det, bool rec, bool amc, int ncu, int max_blocks, size_t lds,
+                                 hipStream_t st, const TraceArgs& P)
 {
+    if (gen) return launch_trace_t<SPEC, TAB, 0, false, true>(amc, ncu, max_blocks, lds, st, P);
     if (rec) return launch_trace_t<SPEC, TAB, 0, true>(amc, ncu, max_blocks, lds, st, P);
     if (det) return launch_trace_t<SPEC, TAB, 1, false>(amc, ncu, max_blocks, lds, st, P);
     return launch_trace_t<SPEC, TAB, 0, false>(amc, ncu, max_blocks, lds, st, P);
 }
 
-static hipError_t launch_trace(int spec, int tab, bool det, bool rec, bool amc, int ncu, int max_blocks, size_t lds,
-                               hipStream_t st, const TraceArgs& P)
+static hipError_t launch_trace(int spec, int tab, bool gen, bool det, bool rec, bool amc, int ncu, int max_blocks,
+                               size_t lds, hipStream_t st, const TraceArgs& P)
 {
-    if (spec == kSpecGreyT4) return launch_trace_m<kSpecGreyT4, kTabGlobal>(det, rec, amc, ncu, max_blocks, lds, st, P);
-    if (tab == kTabSliced) return launch_trace_m<kSpecTable, kTabSliced>(det, rec, amc, ncu, max_blocks, lds, st, P);
-    if (tab == kTabIbLds) return launch_trace_m<kSpecTable, kTabIbLds>(det, rec, amc, ncu, max_blocks, lds, st, P);
-    return launch_trace_m<kSpecTable, kTabGlobal>(det, rec, amc, ncu, max_blocks, lds, st, P);
+    if (spec == kSpecGreyT4)
+        return launch_trace_m<kSpecGreyT4, kTabGlobal>(gen, det, rec, amc, ncu, max_blocks, lds, st, P);
+    if (tab == kTabSliced) return launch_trace_m<kSpecTable, kTabSliced>(gen, det, rec, amc, ncu, max_blocks, lds, st, P);
+    if (tab == kTabIbLds) return launch_trace_m<kSpecTable, kTabIbLds>(gen, det, rec, amc, ncu, max_blocks, lds, st, P);
+    return launch_trace_m<kSpecTable, kTabGlobal>(gen, det, rec, amc, ncu, max_blocks, lds, st, P);
 }
 
 static MlpWeights mlp_weights(dnsmc_solver* s)
@@ -621,8 +623,14 @@ static int do_run(dnsmc_solver* s)
     if (c.mode == DNSMC_MODE_LEARNED && s->have_mlp)   // CDF rows: at most 8 GB per pipeline
         slots_per_batch = std::max<long long>(1, std::min<long long>(slots_per_batch,
                                                                      (8ll << 30) / (4ll * mlp_noutp(s->mlp_nbin))));
-    const long long batch_rays = std::min(slots_per_batch, nslots) * R;
-    const long long nbatch_total = ((nslots + slots_per_batch - 1) / slots_per_batch) * c.nsnap;
+    // plain estimator with atomic tallies and no records: rays are drawn inside
+    // the traversal, one launch for all snapshots (DNSMC_FUSED=0: batched)
+    bool fused = c.mode == DNSMC_MODE_PLAIN && c.precision == 0 && c.tally == DNSMC_TALLY_F64_ATOMIC && !s->record &&
+                 nslots * (long long)R <= (1ll << 32) - (1ll << 20);
+    if (const char* fe = getenv("DNSMC_FUSED")) fused = fused && atoi(fe) != 0;
+    if (fused) slots_per_batch = nslots;
+    const long long batch_rays = fused ? 0 : std::min(slots_per_batch, nslots) * R;
+    const long long nbatch_total = fused ? 1 : ((nslots + slots_per_batch - 1) / slots_per_batch) * c.nsnap;
     int pipes_max = dnsmc_solver::kPipes;   // DNSMC_PIPES=1 serialises the batches (standalone kernel timing)
     if (const char* pe = getenv("DNSMC_PIPES")) pipes_max = std::max(1, std::min(dnsmc_solver::kPipes, atoi(pe)));
     const int npipes = (int)std::min<long long>(pipes_max, nbatch_total);
@@ -710,7 +718,8 @@ static int do_run(dnsmc_solver* s)
     int tab = kTabGlobal;
     P.tab_lo = 0;
     P.tab_nw = d.nT;
-    size_t lds_trace = sizeof(float) * (6 * kLevWords + P.nfaces + P.noffs + P.ncent + 6 * (size_t)d.nB);
+    size_t lds_trace = sizeof(float) * (6 * kLevWords + P.nfaces + P.noffs + P.ncent + 6 * (size_t)d.nB +
+                                        (fused ? (size_t)d.nB : 0));
     if (spec == kSpecTable) {
         float tlo = s->Tlo_snap[0], thi = s->Thi_snap[0];
         for (int i = 1; i < c.nsnap; ++i) {
@@ -814,8 +823,42 @@ static int do_run(dnsmc_solver* s)
 
     const int trace_blocks = s->ncu * 16;
     s->first_batch_rays = 0;
+    P.nsnap_launch = 1;
+    P.tally_stride = 0;
+    if (fused) {
+        // one persistent launch: every snapshot, rays drawn in the refill
+        dnsmc_solver::Pipe& pp = s->pipe[0];
+        hipStream_t st = pp.stream;
+        P.nsnap_launch = c.nsnap;
+        P.tally_stride = (int)nslots;
+        P.nrays = nslots * R;
+        P.slot0 = 0;
+        P.snap = 0;
+        P.seed = c.seed;
+        P.cdf_band = s->d_cdfb.p;
+        P.cdf_g = s->d_cdfg.p;
+        P.inv_emax = s->d_inv_emax.p;
+        P.Ibmax = d.Ibmax;
+        P.g_search = d.g_search;
+        P.w0 = nullptr; P.w1 = nullptr; P.w = nullptr; P.winv = nullptr;
+        P.pray = nullptr;
+        P.E = s->d_E.p;
+        P.A = s->d_A.p;
+        P.Q2 = s->d_Q2.p;
+        P.nrec = 0;
+        P.claim = pp.d_claim.p;
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, pp.d_claim.p, 256ll, 0u);
+        HIPCHK(hipGetLastError());
+        const long long nsc = ((P.nrays + P.super - 1) / P.super) * c.nsnap;
+        const int tb_cap = (int)std::min<long long>(trace_blocks, std::max<long long>(8, (nsc + 15) / 16));
+        EventPair& et = next_event(s, 1);
+        HIPCHK(hipEventRecord(et.a, st));
+        HIPCHK(launch_trace(spec, tab, true, false, false, amc, s->ncu, tb_cap, lds_trace, st, P));
+        HIPCHK(hipEventRecord(et.b, st));
+        s->st.n_trace_launches++;
+    }
     long long bi = 0;
-    for (int snap = 0; snap < c.nsnap; ++snap) {
+    for (int snap = 0; snap < c.nsnap && !fused; ++snap) {
         for (long long b0 = 0; b0 < nslots; b0 += slots_per_batch, ++bi) {
             dnsmc_solver::Pipe& pp = s->pipe[bi % npipes];
             hipStream_t st = pp.stream;
@@ -876,7 +919,7 @@ static int do_run(dnsmc_solver* s)
                                         : (rec ? trace_f64<kSpecTable, true> : trace_f64<kSpecTable, false>);
                 hipLaunchKernelGGL(k, dim3(nb), dim3(256), lds64, st, P, pp.d_pray64.p);
             } else {
-                HIPCHK(launch_trace(spec, tab, det, rec, amc, s->ncu, tb_cap, lds_trace, st, P));
+                HIPCHK(launch_trace(spec, tab, false, det, rec, amc, s->ncu, tb_cap, lds_trace, st, P));
             }
             HIPCHK(hipEventRecord(et.b, st));
             s->st.n_trace_launches++;

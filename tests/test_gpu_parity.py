@@ -348,3 +348,36 @@ def test_fp64_traversal_matches_fp64_oracle(oracle, ck_table_dir, spectral):
     r32 = S32.results(0)
     assert np.max(np.abs(r32["E"] - E) / np.abs(E)) < 1e-5
     assert np.max(np.abs(r32["Q"] - (E - A)) / np.abs(E)) < 1e-5
+
+
+@pytest.mark.parametrize("spectral,grid_num", [(0, 1), (0, 2), (3, 1)])
+def test_fused_generation_matches_batched(ck_table_dir, monkeypatch, spectral, grid_num):
+    """The plain estimator's rays drawn inside the traversal (one launch for
+    all snapshots) against descriptors drawn by raygen_plain (batched
+    launches, which the oracle pins ray by ray): same rays, so equal step
+    totals and tallies equal up to the order of the FP64 atomic additions."""
+    tdir, _ = ck_table_dir
+    n, R, ns = (16, 24, 16), 48, 3
+    maxs = (6, 100000, 100000, 100000, 100000, 100000)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("DNSMC_FUSED", fused)
+        cfg = workloads.channel_config(n, rays_per_cell=R, spectral=spectral, Tw=573.0 if spectral == 0 else 300.0,
+                                       table_dir=tdir, nsnap=ns, grid_num=grid_num, maxs=maxs,
+                                       batch_rays=R * 1000)
+        S = dnsmc_amd.Solver(cfg)
+        S.load_tables()
+        for s in range(ns):
+            T = workloads.field_fourier(60 + s, n) if spectral == 0 else workloads.field_grey32(60 + s, n)
+            S.set_field(s, workloads.ghosted(T))
+        S.run()
+        st = S.stats()
+        out[fused] = (st, [S.results(s) for s in range(ns)])
+        S.close()
+    (sf, rf), (sb, rb) = out["1"], out["0"]
+    assert sf["n_trace_launches"] == 1 and sb["n_trace_launches"] > ns
+    assert sf["n_steps"] == sb["n_steps"]
+    for s in range(ns):
+        np.testing.assert_allclose(rf[s]["E"], rb[s]["E"], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(rf[s]["A"], rb[s]["A"], rtol=1e-12, atol=0)
+        assert np.all(np.abs(rf[s]["Q"] - rb[s]["Q"]) <= 1e-12 * np.abs(rb[s]["E"]))
