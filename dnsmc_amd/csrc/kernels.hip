@@ -776,42 +776,44 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         ++cnt;
         ++rsteps;
         Tc = Tn;   // (read on every path, so the load is issued here, not sunk)
+        // transport, branch-free on the common path: past the cut-off the
+        // remaining energy is deposited here (tra), else the segment absorbs
+        // tra - tra1; a wall absorbs what is left; the step cap ends the ray
+        // (or moves it to the next coarser level)
         const float tau1 = fmaf(kap, ds, tau);
-        if (tau1 >= P.tau_cut) {
-            G = fmaf(tra, ibj, G);
-            alive = false; fin = true; endt = 0; Dfin = 1.0f;
-        } else {
-            const float tra1 = __expf(-tau1);
-            G = fmaf(tra - tra1, ibj, G);
-            tau = tau1;
-            tra = tra1;
-            if (wall >= 0) {
-                G = fmaf(tra, s_ibw[wall * P.nB + band], G);
-                alive = false; fin = true; endt = 1; Dfin = 1.0f;
-            } else if (cnt >= maxs) {
-                if (AMC && lev + 1 < P.nlev) {
-                    // adaptive mesh coarsening (code/mc_gpu.cu:193-207)
-                    const int o0 = n0, o1 = n1, o2 = n2;
-                    ++lev;
-                    const int* L = s_lev + lev * kLevWords;
-                    n0 = L[0]; n1 = L[1]; n2 = L[2];
-                    fo0 = L[3]; fo1 = L[4]; fo2 = L[5]; maxs = L[6];
-                    oo0 = L[11]; oo1 = L[12]; oo2 = L[13];
-                    Tl = lev_T(L);
-                    if (GEN) Tl += (size_t)rsnap * (uint32_t)L[14];
-                    const int q0 = o0 / n0, q1 = o1 / n1, q2 = o2 / n2;
-                    cx = (cx + q0 - 1) / q0;
-                    cy = (cy + q1 - 1) / q1;
-                    cz = (cz + q2 - 1) / q2;
-                    fx = s_face[fo0 + cx - (fl & 1)];
-                    fy = s_face[fo1 + cy - ((fl >> 1) & 1)];
-                    fz = s_face[fo2 + cz - ((fl >> 2) & 1)];
-                    idx = s_off[oo0 + cx - 1] + s_off[oo1 + cy - 1] + s_off[oo2 + cz - 1];
-                    Tc = load_T(Tl, idx);
-                    cnt = 0;
-                } else {
-                    alive = false; fin = true; endt = 2; Dfin = 1.0f - tra;
-                }
+        const bool cut = tau1 >= P.tau_cut;
+        const float tra1 = __expf(-tau1);
+        G = fmaf(cut ? tra : tra - tra1, ibj, G);
+        tau = tau1;
+        tra = cut ? tra : tra1;
+        const bool hitw = !cut && wall >= 0;
+        if (hitw) G = fmaf(tra, s_ibw[wall * P.nB + band], G);
+        const bool cap = !cut && !hitw && cnt >= maxs;
+        if (cut || hitw) {
+            alive = false; fin = true; endt = cut ? 0 : 1; Dfin = 1.0f;
+        } else if (cap) {
+            if (AMC && lev + 1 < P.nlev) {
+                // adaptive mesh coarsening (code/mc_gpu.cu:193-207)
+                const int o0 = n0, o1 = n1, o2 = n2;
+                ++lev;
+                const int* L = s_lev + lev * kLevWords;
+                n0 = L[0]; n1 = L[1]; n2 = L[2];
+                fo0 = L[3]; fo1 = L[4]; fo2 = L[5]; maxs = L[6];
+                oo0 = L[11]; oo1 = L[12]; oo2 = L[13];
+                Tl = lev_T(L);
+                if (GEN) Tl += (size_t)rsnap * (uint32_t)L[14];
+                const int q0 = o0 / n0, q1 = o1 / n1, q2 = o2 / n2;
+                cx = (cx + q0 - 1) / q0;
+                cy = (cy + q1 - 1) / q1;
+                cz = (cz + q2 - 1) / q2;
+                fx = s_face[fo0 + cx - (fl & 1)];
+                fy = s_face[fo1 + cy - ((fl >> 1) & 1)];
+                fz = s_face[fo2 + cz - ((fl >> 2) & 1)];
+                idx = s_off[oo0 + cx - 1] + s_off[oo1 + cy - 1] + s_off[oo2 + cz - 1];
+                Tc = load_T(Tl, idx);
+                cnt = 0;
+            } else {
+                alive = false; fin = true; endt = 2; Dfin = 1.0f - tra;
             }
         }
     }
