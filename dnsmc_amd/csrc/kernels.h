@@ -93,6 +93,7 @@ struct TraceArgs {
     const float* inv_emax;
     float Ibmax;
     int g_search;
+    int gen_lds;                // cdf_g and inv_emax staged in LDS too
     // FP64 traversal extras
     const double* inv_emax_d;
     double Ibmax_d;

@@ -461,6 +461,8 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     float* s_kq = s_ib + P.nB * nw;             // sliced kappa rows (kTabSliced)
     // GEN: band CDF after the tables that are in LDS
     float* s_cdfb = TAB == kTabSliced ? s_kq + P.nB * P.nQ * nw : (TAB == kTabIbLds ? s_ib + P.nB * P.nT : s_ib);
+    float* s_cdfg = s_cdfb + P.nB;              // GEN with gen_lds: g CDF rows and 1/(k Ib)(Tmax)
+    float* s_iem = s_cdfg + P.nB * P.nQ;
     for (int i = threadIdx.x; i < P.nlev; i += blockDim.x) {
         int* L = s_lev + i * kLevWords;
         L[0] = P.lev[i].n[0]; L[1] = P.lev[i].n[1]; L[2] = P.lev[i].n[2];
@@ -481,8 +483,14 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     for (int i = threadIdx.x; i < P.noffs; i += blockDim.x) s_off[i] = P.offs[i];
     for (int i = threadIdx.x; i < P.ncent; i += blockDim.x) s_cent[i] = P.centres[i];
     for (int i = threadIdx.x; i < 6 * P.nB; i += blockDim.x) s_ibw[i] = P.Ibw[i];
-    if (GEN)
+    if (GEN) {
         for (int i = threadIdx.x; i < P.nB; i += blockDim.x) s_cdfb[i] = P.cdf_band[i];
+        if (P.gen_lds)
+            for (int i = threadIdx.x; i < P.nB * P.nQ; i += blockDim.x) {
+                s_cdfg[i] = P.cdf_g[i];
+                s_iem[i] = P.inv_emax[i];
+            }
+    }
     if (SPEC == kSpecTable && TAB == kTabIbLds)
         for (int i = threadIdx.x; i < P.nB * P.nT; i += blockDim.x) s_ib[i] = P.Ib[i];
     if (SPEC == kSpecTable && TAB == kTabSliced) {
@@ -619,7 +627,12 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                     const u32x4 x = philox4x32_10(u32x4{r, (uint32_t)si.y, (uint32_t)(P.snap + (int)rsnap), 0u},
                                                   (uint32_t)P.seed, (uint32_t)(P.seed >> 32));
                     band = cdf_search(s_cdfb, P.nB, u01_open(x.x));
-                    g = P.g_search ? cdf_search(P.cdf_g + band * P.nQ, P.nQ, u01_open(x.y)) : 0;
+                    if (!P.g_search)
+                        g = 0;
+                    else if (P.gen_lds)
+                        g = cdf_search(s_cdfg + band * P.nQ, P.nQ, u01_open(x.y));
+                    else
+                        g = cdf_search(P.cdf_g + band * P.nQ, P.nQ, u01_open(x.y));
                     g = g > P.nQ - 1 ? P.nQ - 1 : g;
                     float kap, ib;
                     if (SPEC == kSpecGreyT4) {
@@ -640,7 +653,8 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                         kap = fmaf(fr, k1 - k0, k0);
                         ib = fmaf(fr, i1 - i0, i0);
                     }
-                    rw = P.Ibmax * (kap * ib) * P.inv_emax[band * P.nQ + g];
+                    const float iem = P.gen_lds ? s_iem[band * P.nQ + g] : P.inv_emax[band * P.nQ + g];
+                    rw = P.Ibmax * (kap * ib) * iem;
                     rwi = rw * (1.0f / ib);
                     mt = x.z >> 8;
                     mp = x.w >> 8;

@@ -747,6 +747,13 @@ static int do_run(dnsmc_solver* s)
             lds_trace += sizeof(float) * (size_t)d.nB * d.nT;
         }
     }
+    // fused generation: the g CDF rows and 1/(k Ib)(Tmax) join the band CDF in
+    // LDS when the block's share still holds them
+    P.gen_lds = 0;
+    if (fused && spec == kSpecTable && lds_trace + 2 * sizeof(float) * (size_t)d.nB * d.nQ <= s->lds_max) {
+        P.gen_lds = 1;
+        lds_trace += 2 * sizeof(float) * (size_t)d.nB * d.nQ;
+    }
     P.rays_per_slot = R;
     P.slots_per_emitter = spe;
     P.fd_R = make_fastdiv((uint32_t)R);
