@@ -758,12 +758,14 @@ static int do_run(dnsmc_solver* s)
     P.slots_per_emitter = spe;
     P.fd_R = make_fastdiv((uint32_t)R);
     {
+        // measured defaults: the fused launch refills at 24 idle lanes and
+        // hands out 512-ray super-chunks; the batched launches 16 and 256
         const char* rk = getenv("DNSMC_REFILL_K");
-        P.refill_k = rk ? atoi(rk) : 16;
+        P.refill_k = rk ? atoi(rk) : (fused ? 24 : 16);
         if (P.refill_k < 1) P.refill_k = 1;
         if (P.refill_k > 64) P.refill_k = 64;
         const char* sc = getenv("DNSMC_SUPER");
-        P.super = sc ? atoi(sc) : 256;
+        P.super = sc ? atoi(sc) : (fused ? 512 : 256);
         P.super = std::max(64, (P.super + 63) / 64 * 64);
     }
     P.slotinfo = s->d_slotinfo.p;
