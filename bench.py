@@ -275,8 +275,40 @@ def main():
                            kind="converged FP64 MC (FP64 traversal and tallies, other seed), the oracle's estimator"),
             estimate_from_variance=dict(plain=math.sqrt(np.sum(res_p["sdev"] ** 2) / np.sum(res_p["Q"] ** 2)),
                                         learned=math.sqrt(np.sum(res_l["sdev"] ** 2) / np.sum(res_l["Q"] ** 2))))
-        out["time_to_1pct_rms_s"] = dict(plain=t_p, learned=t_l, speedup_learned=t_p / t_l,
-                                         note="RMS error scales as rays^-1/2; time scales with rays")
+        # rays per cell for 1% RMS (error ~ R^-1/2) and the cost of a solve as a
+        # function of R: fixed part (per-cell proposal, launch, tails) plus a
+        # marginal cost per ray per cell, measured on snapshot 0 at R and 4R and
+        # scaled to the workload's snapshots
+        R1_p = R * (eps_p / 0.01) ** 2
+        R1_l = R * (eps_l / 0.01) ** 2
+        f0 = fields[:1]
+        lin = {}
+        for name, mode, W in (("plain", 0, None), ("learned", 2, weights)):
+            ts = []
+            for RR in (R, 4 * R):
+                S1, _ = study.make_solver(args.workload, RR, mode=mode, mlp=W, mix_eps=args.mix_eps, fields=f0)
+                S1.run()
+                S1.wait()
+                t0 = time.perf_counter()
+                S1.run()
+                S1.wait()
+                ts.append(time.perf_counter() - t0)
+                S1.close()
+            b = (ts[1] - ts[0]) / (3 * R)
+            a = ts[0] - b * R
+            lin[name] = (a, b, ts)
+        ns = cfg.nsnap
+        T1_p = ns * (lin["plain"][0] + lin["plain"][1] * R1_p)
+        T1_l = ns * (lin["learned"][0] + lin["learned"][1] * R1_l)
+        out["time_to_1pct_rms_s"] = dict(
+            plain=T1_p, learned=T1_l, speedup_learned=T1_p / T1_l,
+            rays_per_cell_to_1pct=dict(plain=R1_p, learned=R1_l),
+            solve_s_snapshot0=dict(plain={str(R): lin["plain"][2][0], str(4 * R): lin["plain"][2][1]},
+                                   learned={str(R): lin["learned"][2][0], str(4 * R): lin["learned"][2][1]}),
+            at_bench_rays=dict(plain=t_p, learned=t_l, speedup_learned=t_p / t_l),
+            note="rays per cell to 1%% RMS from the error at %d rays/cell (error ~ R^-1/2); solve time at that "
+                 "R from the measured fixed + per-ray cost on snapshot 0 (R and 4R) times %d snapshots; "
+                 "at_bench_rays scales the whole measured solve linearly instead" % (R, ns))
     print(json.dumps(out), flush=True)
 
 
