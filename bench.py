@@ -32,7 +32,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(workload, rays, nsnap=None, mode=0, batch=None, mlp=None, mix_eps=0.1):
+def setup(workload, rays, nsnap=None, mode=0, batch=None):
     import dnsmc_amd
     from dnsmc_amd import workloads
     w = workloads.WORKLOADS[workload]
@@ -49,8 +49,6 @@ def setup(workload, rays, nsnap=None, mode=0, batch=None, mlp=None, mix_eps=0.1)
     else:
         table = workloads.synth_ck_table(11)
         S.set_tables(workloads.table_arrays(table))
-    if mlp is not None:
-        S.set_mlp(*mlp, mix_eps=mix_eps)
     fields = workloads.workload_fields(workload, ns)
     for s, T in enumerate(fields):
         S.set_field(s, workloads.ghosted(T))
@@ -134,6 +132,76 @@ def timed_solve(S, steps):
     return (time.perf_counter() - t0) / steps, st_all
 
 
+def learned_leg(args, cfg, R, fields, res_p, t_plain, cfg_kw):
+    """Train the proposal, run the learned solve of the workload, and compare
+    both estimators with a converged FP64 reference on 1024 cells of snapshot
+    0: RMS / max error at R rays per cell and the time to 1% RMS."""
+    from dnsmc_amd import learned, study
+    cells = study.sample_cells(args.workload, args.ref_cells)
+    Qref, vref, t_ref = study.reference(args.workload, cells, args.ref_rays, fields=fields, **cfg_kw)
+    weights, info = learned.train_channel_proposal(args.train_snaps, args.train_rays, args.cost0, args.train_epochs,
+                                                   **cfg_kw)
+    log("proposal: %s" % (info,))
+    SL, _ = study.make_solver(args.workload, R, nsnap=cfg.nsnap, mode=2, mlp=weights, mix_eps=args.mix_eps,
+                              fields=fields, **cfg_kw)
+    dt_l, st_l = timed_solve(SL, 1)
+    res_l = SL.results(0)
+    SL.close()
+    err_p = study.errors(res_p["Q"][cells], Qref, vref)
+    err_l = study.errors(res_l["Q"][cells], Qref, vref)
+    eps_p, eps_l = err_p["rms"], err_l["rms"]
+    t_p = t_plain * (eps_p / 0.01) ** 2
+    t_l = dt_l * (eps_l / 0.01) ** 2
+    var_ratio = float(np.sum(res_p["sdev"] ** 2) / np.sum(res_l["sdev"] ** 2))
+    out = {}
+    out["learned"] = dict(info, mix_eps=args.mix_eps, ms_per_step=dt_l * 1000.0,
+                          rays_per_s=st_l[0]["n_rays"] / dt_l,
+                          segments_per_ray=st_l[0]["n_steps"] / max(st_l[0]["n_rays"], 1),
+                          variance_ratio_vs_plain=var_ratio)
+    out["rms_error_vs_fp64_oracle"] = dict(
+        plain=eps_p, learned=eps_l, max_plain=err_p["max"], max_learned=err_l["max"], rays_per_cell=R,
+        snapshot=0, cells=int(len(cells)),
+        reference=dict(rays_per_cell=args.ref_rays, seconds=round(t_ref, 1),
+                       kind="converged FP64 MC (FP64 traversal and tallies, other seed), the oracle's estimator"),
+        estimate_from_variance=dict(plain=math.sqrt(np.sum(res_p["sdev"] ** 2) / np.sum(res_p["Q"] ** 2)),
+                                    learned=math.sqrt(np.sum(res_l["sdev"] ** 2) / np.sum(res_l["Q"] ** 2))))
+    # rays per cell for 1% RMS (error ~ R^-1/2) and the cost of a solve as a
+    # function of R: fixed part (per-cell proposal, launch, tails) plus a
+    # marginal cost per ray per cell, measured on snapshot 0 at R and 4R and
+    # scaled to the workload's snapshots
+    R1_p = R * (eps_p / 0.01) ** 2
+    R1_l = R * (eps_l / 0.01) ** 2
+    f0 = fields[:1]
+    lin = {}
+    for name, mode, W in (("plain", 0, None), ("learned", 2, weights)):
+        ts = []
+        for RR in (R, 4 * R):
+            S1, _ = study.make_solver(args.workload, RR, mode=mode, mlp=W, mix_eps=args.mix_eps, fields=f0, **cfg_kw)
+            S1.run()
+            S1.wait()
+            t0 = time.perf_counter()
+            S1.run()
+            S1.wait()
+            ts.append(time.perf_counter() - t0)
+            S1.close()
+        b = (ts[1] - ts[0]) / (3 * R)
+        a = ts[0] - b * R
+        lin[name] = (a, b, ts)
+    ns = cfg.nsnap
+    T1_p = ns * (lin["plain"][0] + lin["plain"][1] * R1_p)
+    T1_l = ns * (lin["learned"][0] + lin["learned"][1] * R1_l)
+    out["time_to_1pct_rms_s"] = dict(
+        plain=T1_p, learned=T1_l, speedup_learned=T1_p / T1_l,
+        rays_per_cell_to_1pct=dict(plain=R1_p, learned=R1_l),
+        solve_s_snapshot0=dict(plain={str(R): lin["plain"][2][0], str(4 * R): lin["plain"][2][1]},
+                               learned={str(R): lin["learned"][2][0], str(4 * R): lin["learned"][2][1]}),
+        at_bench_rays=dict(plain=t_p, learned=t_l, speedup_learned=t_p / t_l),
+        note="rays per cell to 1%% RMS from the error at %d rays/cell (error ~ R^-1/2); solve time at that "
+             "R from the measured fixed + per-ray cost on snapshot 0 (R and 4R) times %d snapshots; "
+             "at_bench_rays scales the whole measured solve linearly instead" % (R, ns))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +215,7 @@ def main():
     ap.add_argument("--batch", type=float, default=None, help="rays per traversal launch")
     ap.add_argument("--copy-gib", type=float, default=1.0, help="size of the achievable-bandwidth copy (0: skip)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg and the error study")
+    ap.add_argument("--no-amc", action="store_true", help="skip the leg with the reference's default mesh coarsening")
     ap.add_argument("--train-snaps", type=int, default=20)
     ap.add_argument("--train-rays", type=int, default=8)
     ap.add_argument("--train-epochs", type=int, default=2)
@@ -205,7 +274,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "grid": list(cfg.n), "snapshots": cfg.nsnap, "rays_per_cell": R,
                    "table": "grey kappa=1" if table is None else "synthetic c-k 100x16x69",
-                   "estimator": "plain (Tmax proposal, Philox)", "tally": "fp64", "tables": tables},
+                   "estimator": "plain (Tmax proposal, Philox)", "tally": "fp64", "tables": tables,
+                   "coarsening": "none (grid_num 1); reference_amc repeats the comparison with the reference's "
+                                 "default 5-level coarsening"},
         "segments_per_s": nsteps / dt,
         "segments_per_ray": nsteps / max(nrays, 1),
         "ms_trace_per_step": ms_trace / args.steps,
@@ -243,72 +314,24 @@ def main():
         except Exception as e:  # reported, never fatal to the GPU number
             log("cpu baseline failed: %r" % (e,))
     if not args.no_learned:
-        # time to 1% RMS error of Q_rad against a converged FP64 reference,
-        # plain vs learned proposal, at the same rays per cell
-        from dnsmc_amd import learned, study
         res_p = S.results(0)
         S.close()
-        cells = study.sample_cells(args.workload, args.ref_cells)
-        Qref, vref, t_ref = study.reference(args.workload, cells, args.ref_rays, fields=fields)
-        weights, info = learned.train_channel_proposal(args.train_snaps, args.train_rays, args.cost0,
-                                                       args.train_epochs)
-        log("proposal: %s" % (info,))
-        SL, cfgl, _, _, _ = setup(args.workload, args.rays, args.nsnap, mode=2, batch=args.batch, mlp=weights,
-                                  mix_eps=args.mix_eps)
-        dt_l, st_l = timed_solve(SL, 1)
-        res_l = SL.results(0)
-        SL.close()
-        err_p = study.errors(res_p["Q"][cells], Qref, vref)
-        err_l = study.errors(res_l["Q"][cells], Qref, vref)
-        eps_p, eps_l = err_p["rms"], err_l["rms"]
-        t_p = ms_step / 1000.0 * (eps_p / 0.01) ** 2
-        t_l = dt_l * (eps_l / 0.01) ** 2
-        var_ratio = float(np.sum(res_p["sdev"] ** 2) / np.sum(res_l["sdev"] ** 2))
-        out["learned"] = dict(info, mix_eps=args.mix_eps, ms_per_step=dt_l * 1000.0,
-                              rays_per_s=st_l[0]["n_rays"] / dt_l,
-                              segments_per_ray=st_l[0]["n_steps"] / max(st_l[0]["n_rays"], 1),
-                              variance_ratio_vs_plain=var_ratio)
-        out["rms_error_vs_fp64_oracle"] = dict(
-            plain=eps_p, learned=eps_l, max_plain=err_p["max"], max_learned=err_l["max"], rays_per_cell=R,
-            snapshot=0, cells=int(len(cells)),
-            reference=dict(rays_per_cell=args.ref_rays, seconds=round(t_ref, 1),
-                           kind="converged FP64 MC (FP64 traversal and tallies, other seed), the oracle's estimator"),
-            estimate_from_variance=dict(plain=math.sqrt(np.sum(res_p["sdev"] ** 2) / np.sum(res_p["Q"] ** 2)),
-                                        learned=math.sqrt(np.sum(res_l["sdev"] ** 2) / np.sum(res_l["Q"] ** 2))))
-        # rays per cell for 1% RMS (error ~ R^-1/2) and the cost of a solve as a
-        # function of R: fixed part (per-cell proposal, launch, tails) plus a
-        # marginal cost per ray per cell, measured on snapshot 0 at R and 4R and
-        # scaled to the workload's snapshots
-        R1_p = R * (eps_p / 0.01) ** 2
-        R1_l = R * (eps_l / 0.01) ** 2
-        f0 = fields[:1]
-        lin = {}
-        for name, mode, W in (("plain", 0, None), ("learned", 2, weights)):
-            ts = []
-            for RR in (R, 4 * R):
-                S1, _ = study.make_solver(args.workload, RR, mode=mode, mlp=W, mix_eps=args.mix_eps, fields=f0)
-                S1.run()
-                S1.wait()
-                t0 = time.perf_counter()
-                S1.run()
-                S1.wait()
-                ts.append(time.perf_counter() - t0)
-                S1.close()
-            b = (ts[1] - ts[0]) / (3 * R)
-            a = ts[0] - b * R
-            lin[name] = (a, b, ts)
-        ns = cfg.nsnap
-        T1_p = ns * (lin["plain"][0] + lin["plain"][1] * R1_p)
-        T1_l = ns * (lin["learned"][0] + lin["learned"][1] * R1_l)
-        out["time_to_1pct_rms_s"] = dict(
-            plain=T1_p, learned=T1_l, speedup_learned=T1_p / T1_l,
-            rays_per_cell_to_1pct=dict(plain=R1_p, learned=R1_l),
-            solve_s_snapshot0=dict(plain={str(R): lin["plain"][2][0], str(4 * R): lin["plain"][2][1]},
-                                   learned={str(R): lin["learned"][2][0], str(4 * R): lin["learned"][2][1]}),
-            at_bench_rays=dict(plain=t_p, learned=t_l, speedup_learned=t_p / t_l),
-            note="rays per cell to 1%% RMS from the error at %d rays/cell (error ~ R^-1/2); solve time at that "
-                 "R from the measured fixed + per-ray cost on snapshot 0 (R and 4R) times %d snapshots; "
-                 "at_bench_rays scales the whole measured solve linearly instead" % (R, ns))
+        out.update(learned_leg(args, cfg, R, fields, res_p, ms_step / 1000.0, {}))
+        if not args.no_amc:
+            # the same comparison with the reference's default adaptive mesh
+            # coarsening (code/param.h: 5 levels, 5 steps on each of the first four)
+            amc = dict(grid_num=5, maxs=(5, 5, 5, 5, 100000, 100000))
+            from dnsmc_amd import study
+            SA, _ = study.make_solver(args.workload, R, nsnap=cfg.nsnap, fields=fields, **amc)
+            dt_a, st_a = timed_solve(SA, 1)
+            res_a = SA.results(0)
+            SA.close()
+            leg = learned_leg(args, cfg, R, fields, res_a, dt_a, amc)
+            out["reference_amc"] = dict(
+                config="grid_num 5, maxs 5 5 5 5 100000 (the reference's code/param.h defaults), else as above",
+                plain=dict(rays_per_s=st_a[0]["n_rays"] / dt_a, ms_per_step=dt_a * 1000.0,
+                           segments_per_ray=st_a[0]["n_steps"] / max(st_a[0]["n_rays"], 1)),
+                **leg)
     print(json.dumps(out), flush=True)
 
 

@@ -154,10 +154,12 @@ def train(net, samples, epochs=3, batch=8192, lr=2e-3, seed=0, log=None):
     return hist
 
 
-def train_channel_proposal(n_snap=20, rays=8, cost0=16.0, epochs=2, batch=16384, lr=2e-3, log=None):
+def train_channel_proposal(n_snap=20, rays=8, cost0=16.0, epochs=2, batch=16384, lr=2e-3, log=None, **cfg_kw):
     """The channel's proposal: plain-run tallies of `n_snap` seeded 64x96x64
     snapshots (parabolic profile 573-955 K + N(0, 10 K), seeds 31..) with the
     synthetic 100x16x69 table, `rays` rays per cell, then `epochs` of Adam.
+    cfg_kw: further configuration fields of the training runs (the proposal
+    should be trained with the traversal it will serve, e.g. grid_num, maxs).
     Returns (weights for Solver.set_mlp, info dict)."""
     import time
     from . import workloads
@@ -168,7 +170,7 @@ def train_channel_proposal(n_snap=20, rays=8, cost0=16.0, epochs=2, batch=16384,
     samples = SampleSet()
     nbin = None
     for s in range(n_snap):
-        S = Solver(workloads.channel_config(n1, rays_per_cell=rays, spectral=0, Tw=573.0))
+        S = Solver(workloads.channel_config(n1, rays_per_cell=rays, spectral=0, Tw=573.0, **cfg_kw))
         S.set_tables(workloads.table_arrays(table))
         S.set_record(True)
         S.set_field(0, workloads.ghosted(workloads.field_profile_noise(31 + s, n1)))

@@ -31,10 +31,11 @@ def log(*a):
 
 
 def make_solver(workload, rays, nsnap=1, mode=0, precision=0, tally=0, seed=None, cells=None, mlp=None,
-                mix_eps=0.1, batch=None, fields=None):
-    """Solver of a named channel workload with its table and seeded snapshots."""
+                mix_eps=0.1, batch=None, fields=None, **cfg_kw):
+    """Solver of a named channel workload with its table and seeded snapshots
+    (cfg_kw: further configuration fields, e.g. grid_num and maxs)."""
     w = workloads.WORKLOADS[workload]
-    kw = dict(mode=mode, precision=precision, tally=tally)
+    kw = dict(cfg_kw, mode=mode, precision=precision, tally=tally)
     if seed is not None:
         kw["seed"] = seed
     if batch:
@@ -64,11 +65,11 @@ def sample_cells(workload, k, key=11):
     return np.sort(rng.choice(ncell, size=min(k, ncell), replace=False)).astype(np.int32)
 
 
-def reference(workload, cells, rays_ref, seed=987654321, fields=None):
+def reference(workload, cells, rays_ref, seed=987654321, fields=None, **cfg_kw):
     """Converged FP64 reference MC of snapshot 0 on `cells`: (Q, var(Q), seconds)."""
     t0 = time.time()
     S, _ = make_solver(workload, rays_ref, precision=1, tally=1, seed=seed, cells=cells,
-                       fields=None if fields is None else fields[:1])
+                       fields=None if fields is None else fields[:1], **cfg_kw)
     S.run()
     r = S.results(0)
     S.close()
