@@ -184,3 +184,32 @@ def test_learned_estimator_is_unbiased_and_reduces_variance(trained):
     assert np.max(np.abs(zrow)) < 4.5, zrow
     ratio = np.sum(var_p) / np.sum(var_l)
     assert ratio > 3.0, ratio
+
+
+def test_learned_estimator_unbiased_with_coarsening(trained):
+    """The learned generator with two coarsening levels (the batched traversal
+    through the coarse field): the p/q-reweighted estimator agrees with the
+    plain one on the same configuration (any proposal is unbiased)."""
+    net, _, tdir, n = trained
+    field = _field(52, n)
+    kw = dict(grid_num=2, maxs=(4, 100000, 100000, 100000, 100000, 100000))
+    out = []
+    for mode, seed, W in ((0, 303, None), (2, 404, net)):
+        cfg = workloads.channel_config(n, rays_per_cell=256, spectral=0, Tw=573.0, table_dir=tdir, mode=mode, **kw)
+        cfg.seed = seed
+        S = dnsmc_amd.Solver(cfg)
+        S.load_tables()
+        if W is not None:
+            S.set_mlp(*W.host_weights(), mix_eps=0.1)
+        S.set_field(0, field)
+        S.run()
+        out.append(S.results(0))
+        S.close()
+    plain, lrn = out
+    var = plain["sdev"] ** 2 + lrn["sdev"] ** 2
+    dq = lrn["Q"] - plain["Q"]
+    ztot = np.sum(dq) / np.sqrt(np.sum(var))
+    assert abs(ztot) < 4.0, ztot
+    rows = dq.reshape(n[2], n[1], n[0]).sum(axis=(0, 2)) / np.sqrt(var.reshape(n[2], n[1], n[0]).sum(axis=(0, 2)))
+    assert np.max(np.abs(rows)) < 4.5, rows
+    assert np.sum(plain["sdev"] ** 2) / np.sum(lrn["sdev"] ** 2) > 2.0

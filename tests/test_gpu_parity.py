@@ -381,3 +381,23 @@ def test_fused_generation_matches_batched(ck_table_dir, monkeypatch, spectral, g
         np.testing.assert_allclose(rf[s]["E"], rb[s]["E"], rtol=1e-12, atol=0)
         np.testing.assert_allclose(rf[s]["A"], rb[s]["A"], rtol=1e-12, atol=0)
         assert np.all(np.abs(rf[s]["Q"] - rb[s]["Q"]) <= 1e-12 * np.abs(rb[s]["E"]))
+
+
+def test_fp64_traversal_with_coarsening_matches_fp64_oracle(oracle, ck_table_dir):
+    """precision 1 through two coarsening levels (5 fine steps, then the
+    coarse spline field): tallies against the oracle's FP64 estimator."""
+    tdir, _ = ck_table_dir
+    n, R = (16, 24, 16), 24
+    cfg = workloads.channel_config(n, rays_per_cell=R, spectral=0, Tw=573.0, table_dir=tdir, precision=1, tally=1,
+                                   grid_num=2, maxs=(5, 100000, 100000, 100000, 100000, 100000))
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    S.set_field(0, workloads.ghosted(workloads.field_fourier(9, n)))
+    S.run()
+    st = S.stats()
+    res = S.results(0)
+    P = oracle.problem(oracle_problem_dict(S, cfg))
+    E, A, Q2, ns = oracle.mc_philox(P, cfg.seed, 0, R, f64=True)
+    np.testing.assert_allclose(res["E"], E, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(res["A"], A, rtol=1e-12, atol=0)
+    assert st["n_steps"] == ns
