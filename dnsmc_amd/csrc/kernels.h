@@ -94,6 +94,7 @@ struct TraceArgs {
     float Ibmax;
     int g_search;
     int gen_lds;                // cdf_g and inv_emax staged in LDS too
+    int desc_ratio;             // learned descriptors: w holds p/q, the weight is computed at ray start
     // FP64 traversal extras
     const double* inv_emax_d;
     double Ibmax_d;

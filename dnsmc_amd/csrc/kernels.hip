@@ -621,6 +621,8 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 Tc = T0[si.z];
                 int g;
                 uint32_t mt, mp;
+                float ratio = 1.0f;       // p/q of a learned descriptor
+                bool weigh = GEN;         // emission weight computed here (tables in LDS)
                 if (GEN) {
                     // the plain draw of ray r of the cell (raygen_plain)
                     const uint32_t r = q - slot * (uint32_t)P.rays_per_slot;
@@ -634,6 +636,25 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                     else
                         g = cdf_search(P.cdf_g + band * P.nQ, P.nQ, u01_open(x.y));
                     g = g > P.nQ - 1 ? P.nQ - 1 : g;
+                    mt = x.z >> 8;
+                    mp = x.w >> 8;
+                } else {
+                    const uint32_t a0 = P.w0[q], a1 = P.w1[q];
+                    band = desc_band(a0, a1);
+                    g = desc_g(a1);
+                    mt = a0 & 0xFFFFFFu;
+                    mp = a1 & 0xFFFFFFu;
+                    if (P.desc_ratio) {
+                        ratio = P.w[q];
+                        weigh = true;
+                    } else {
+                        rw = P.w[q];
+                        rwi = P.winv[q];
+                    }
+                }
+                if (weigh) {
+                    // Ibmax (k_i Ib_i) / (k_max Ib_max) (code/mc_gpu.cu:63, :163-169),
+                    // the arithmetic of the generators' bundle weights
                     float kap, ib;
                     if (SPEC == kSpecGreyT4) {
                         const float t2 = Tc * Tc;
@@ -654,18 +675,10 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                         ib = fmaf(fr, i1 - i0, i0);
                     }
                     const float iem = P.gen_lds ? s_iem[band * P.nQ + g] : P.inv_emax[band * P.nQ + g];
-                    rw = P.Ibmax * (kap * ib) * iem;
-                    rwi = rw * (1.0f / ib);
-                    mt = x.z >> 8;
-                    mp = x.w >> 8;
-                } else {
-                    const uint32_t a0 = P.w0[q], a1 = P.w1[q];
-                    rw = P.w[q];
-                    rwi = P.winv[q];
-                    band = desc_band(a0, a1);
-                    g = desc_g(a1);
-                    mt = a0 & 0xFFFFFFu;
-                    mp = a1 & 0xFFFFFFu;
+                    const float w = P.Ibmax * (kap * ib) * iem;
+                    const float wi = w * (1.0f / ib);
+                    rw = GEN ? w : w * ratio;
+                    rwi = GEN ? wi : wi * ratio;
                 }
                 const float ut = ((float)mt + P.u_off) * kTwoM24;
                 const float up = ((float)mp + P.u_off) * kTwoM24;

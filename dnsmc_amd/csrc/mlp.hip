@@ -474,31 +474,6 @@ __global__ void mlp_features(FeatArgs f, const int4* slotinfo, long long n, floa
     }
 }
 
-// Emission weights (code/mc_gpu.cu:63,:163-169), as in kernels.hip.
-__device__ __forceinline__ void bundle_weight_l(const GenArgs& a, float Ti, int band, int g, float* w, float* winv)
-{
-    float kap, ib;
-    const int bg = band * a.nQ + g;
-    if (a.spec == kSpecGreyT4) {
-        const float t2 = Ti * Ti;
-        kap = a.kappa_grey;
-        ib = t2 * t2;
-    } else {
-        float pos = fmaf(Ti, a.inv_dT, a.mT0);
-        pos = fminf(fmaxf(pos, 0.0f), (float)(a.nT - 1));
-        int t0 = (int)pos;
-        t0 = t0 > a.nT - 2 ? a.nT - 2 : t0;
-        const float fr = pos - (float)t0;
-        const float* kr = a.kq + (size_t)bg * a.nT;
-        kap = fmaf(fr, kr[t0 + 1] - kr[t0], kr[t0]);
-        const float* ir = a.Ib + (size_t)band * a.nT;
-        ib = fmaf(fr, ir[t0 + 1] - ir[t0], ir[t0]);
-    }
-    const float ww = a.Ibmax * (kap * ib) * a.inv_emax[bg];
-    *w = ww;
-    *winv = ww * (1.0f / ib);
-}
-
 // -------------------------------------------------- learned ray generation --
 // mlp_proposal: one block per 64 consecutive slots, one pass over the output
 // tiles.  The network's logits of the 64 cells (MFMA), e = exp(clamp(z)) and
@@ -616,8 +591,9 @@ __device__ __forceinline__ int mix_search(const float* Uh, const float* P, int n
 // raygen_learned: one block per 8 consecutive slots.  Their rows of U are
 // staged in LDS with the plain (band, g) CDF; each ray draws Philox
 // (r, cell, snapshot, 1), its direction bin and (band, g) bin from the
-// defensive mixtures, its direction uniform inside the bin, and carries the
-// weights times p/q.
+// defensive mixtures and its direction uniform inside the bin.  The
+// descriptor's w word carries p/q; the traversal multiplies it into the
+// emission weight it computes from its LDS tables at ray start.
 __global__ __launch_bounds__(256) void raygen_learned(GenArgs a, const float* U, const float* pcdf)
 {
     extern __shared__ float lds[];
@@ -626,6 +602,7 @@ __global__ __launch_bounds__(256) void raygen_learned(GenArgs a, const float* U,
     for (int e = threadIdx.x; e < nbin; e += blockDim.x) sP[e] = pcdf[e];
     __shared__ float sPd[kMlpDir];
     for (int e = threadIdx.x; e < kMlpDir; e += blockDim.x) sPd[e] = (float)(e + 1) * (1.0f / kMlpDir);
+    __shared__ int s_cell[8];
     const float eps = a.mix_eps;
     const int R = a.rays_per_slot;
     const long long ngroups = (a.nslots + 7) / 8;
@@ -636,19 +613,16 @@ __global__ __launch_bounds__(256) void raygen_learned(GenArgs a, const float* U,
         const float4* src = (const float4*)(U + s0 * (size_t)noutp);
         float4* dst = (float4*)lds;
         for (int e = threadIdx.x; e < ns * noutp / 4; e += blockDim.x) dst[e] = src[e];
+        if (threadIdx.x < ns) s_cell[threadIdx.x] = a.slotinfo[a.slot0 + s0 + threadIdx.x].y;
         __syncthreads();
         for (int t = threadIdx.x; t < ns * R; t += blockDim.x) {
             const int sl = t / R, r = t - sl * R;
-            const int4 si = a.slotinfo[a.slot0 + s0 + sl];
-            const u32x4 x = philox4x32_10(u32x4{(uint32_t)r, (uint32_t)si.y, (uint32_t)a.snap, 1u}, k0, k1);
+            const u32x4 x = philox4x32_10(u32x4{(uint32_t)r, (uint32_t)s_cell[sl], (uint32_t)a.snap, 1u}, k0, k1);
             const float* Ud = lds + sl * noutp;
             float qd, qb;
             const int d = mix_search(Ud, sPd, kMlpDir, eps, u01_open(x.x), &qd);
             const int bg = mix_search(Ud + kMlpDir, sP, nbin, eps, u01_open(x.y), &qb);
             const int band = bg / a.nQ, g = bg - band * a.nQ;
-            const float Ti = a.T0[(size_t)a.snap * a.tstride0 + si.z];
-            float w, winv;
-            bundle_weight_l(a, Ti, band, g, &w, &winv);
             // (a selected bin always has a positive step; the guard keeps a
             // degenerate row from producing inf * 0)
             const float ratio = (qb > 0.f && qd > 0.f) ? (a.p_bg[bg] / qb) * ((1.0f / kMlpDir) / qd) : 0.f;
@@ -659,8 +633,7 @@ __global__ __launch_bounds__(256) void raygen_learned(GenArgs a, const float* U,
             const long long q = (s0 + sl) * (long long)R + r;
             a.w0[q] = w0;
             a.w1[q] = w1;
-            a.w[q] = w * ratio;
-            a.winv[q] = winv * ratio;
+            a.w[q] = ratio;
         }
         __syncthreads();
     }

@@ -750,6 +750,9 @@ static int do_run(dnsmc_solver* s)
     // fused generation: the g CDF rows and 1/(k Ib)(Tmax) join the band CDF in
     // LDS when the block's share still holds them
     P.gen_lds = 0;
+    P.desc_ratio = c.mode == DNSMC_MODE_LEARNED ? 1 : 0;
+    P.inv_emax = s->d_inv_emax.p;
+    P.Ibmax = d.Ibmax;
     if (fused && spec == kSpecTable && lds_trace + 2 * sizeof(float) * (size_t)d.nB * d.nQ <= s->lds_max) {
         P.gen_lds = 1;
         lds_trace += 2 * sizeof(float) * (size_t)d.nB * d.nQ;

@@ -178,7 +178,12 @@ int dnsmc_get_stats(dnsmc_solver *s, dnsmc_stats *st);
 /* Derived arrays: name in {"faces0","faces1",..,"centres","T1".."T5","cdf_band","cdf_g",
  * "inv_emax","Ib","kq","Ibw","scalars"}; copies up to n floats, returns count. */
 int dnsmc_get_array(dnsmc_solver *s, const char *name, int level, float *out, long long n);
-/* Ray descriptors of the last run's first batch (SoA words) for parity tests. */
+/* Ray descriptors of the last run's first batch (SoA words) for parity tests.
+ * The plain estimator draws its rays inside the traversal unless records are
+ * on (dnsmc_set_record) or a deterministic tally is chosen, so descriptors
+ * exist only for those runs and for the compat, sorted and learned modes; a
+ * learned descriptor's w holds p/q (the emission weight is applied at ray
+ * start) and its winv is not written. */
 long long dnsmc_get_rays(dnsmc_solver *s, unsigned int *w0, unsigned int *w1, float *w, float *winv,
                          long long n);
 /* Per-ray record of the last run's first batch: steps, end type, hash, D, G. */
