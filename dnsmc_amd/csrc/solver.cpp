@@ -665,7 +665,8 @@ static int do_run(dnsmc_solver* s)
     s->ev_used = 0;
     memset(&s->st, 0, sizeof s->st);
     s->t0 = std::chrono::steady_clock::now();
-    HIPCHK(hipMemsetAsync(s->d_nsteps.p, 0, sizeof(unsigned long long), s->stream));
+    // counters are cleared by fill kernels (stream-ordered, capture-safe)
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s->stream, (unsigned*)s->d_nsteps.p, 2ll, 0u);
     {
         long long n = nslots * c.nsnap;
         int nb = (int)((n + 255) / 256);

@@ -163,7 +163,17 @@ int dnsmc_set_mlp(dnsmc_solver *s, const unsigned short *W1, const float *b1, co
 int dnsmc_set_emitters_slab(dnsmc_solver *s, int ystart);
 int dnsmc_set_emitters_list(dnsmc_solver *s, const int *cells, int ncells);
 
-/* Queue the whole radiation solve (all snapshots) and return. */
+/* Queue the whole radiation solve (all snapshots) and return.
+ * The plain estimator with FP64 atomic tallies and records off draws its
+ * rays inside the traversal, one launch for all snapshots; the other modes
+ * generate descriptors in batches of batch_rays on three pipeline streams.
+ * Environment overrides, read at each run (measurement and diagnosis; the
+ * defaults are the measured best):
+ *   DNSMC_FUSED=0     batched generation for the plain estimator too
+ *   DNSMC_PIPES=1..3  pipeline streams of the batched modes
+ *   DNSMC_TABLES=global  keep kappa / Planck tables out of LDS
+ *   DNSMC_REFILL_K=k  idle lanes of a wave before it starts new rays
+ *   DNSMC_SUPER=n     rays per claimed super-chunk (multiple of 64) */
 int dnsmc_run(dnsmc_solver *s);
 /* Wait for the queued solve. */
 int dnsmc_wait(dnsmc_solver *s);
