@@ -100,6 +100,18 @@ __device__ __forceinline__ void layer1(const float* sX, const MlpWeights& w, int
     }
 }
 
+// Stage column tile jt of W3 ([64][noutp] BF16, row-major) into dst[64][64]:
+// 64 rows of 128 contiguous bytes, 16 B per load.
+__device__ __forceinline__ void stage_w3_tile(const uint16_t* W3, int noutp, int jt, uint16_t* dst)
+{
+    const uint4* src = (const uint4*)W3;
+    uint4* d = (uint4*)dst;
+    for (int e = threadIdx.x; e < 512; e += blockDim.x) {
+        const int h = e >> 3, c8 = e & 7;
+        d[e] = src[((size_t)h * noutp + jt * 64) / 8 + c8];
+    }
+}
+
 // ------------------------------------------------------------- training ----
 // One block per 64 samples: forward, weighted cross-entropy of both heads,
 // backward; the block's gradient of (sum_s wt_s CE_s) / sum wt goes to its row
@@ -112,13 +124,9 @@ __global__ __launch_bounds__(256) void mlp_train_tile(MlpTrainArgs a)
     float* sH2f = sH1f + 4096;                        // 64*64 (later dA1)
     float* sZ = sH2f + 4096;                          // 64*64 logits / dZ / dA2
     float* sw = sZ + 4096;                            // 64 sample scale
-    float* sm_d = sw + 64;                            // row stats
-    float* ss_d = sm_d + 64;
-    float* sm_b = ss_d + 64;
-    float* ss_b = sm_b + 64;
-    float* szt_d = ss_b + 64;
-    float* szt_b = szt_d + 64;
-    float* sloss = szt_b + 64;                        // 64
+    float* sm_d = sw + 64;                            // log-sum-exp per row, direction head
+    float* sm_b = sm_d + 64;                          // log-sum-exp per row, (band, g) head
+    float* sloss = sm_b + 64;                         // 64
     int* std_ = (int*)(sloss + 64);                   // 64
     int* stb_ = std_ + 64;                            // 64
     uint16_t* sH1b = (uint16_t*)(stb_ + 64);          // 64*64
@@ -144,9 +152,6 @@ __global__ __launch_bounds__(256) void mlp_train_tile(MlpTrainArgs a)
         sw[tid] = v ? a.wt[r0 + tid] * inv_w : 0.f;
         std_[tid] = v ? a.td[r0 + tid] : 0;
         stb_[tid] = v ? a.tb[r0 + tid] : 0;
-        sm_d[tid] = -3.0e38f; ss_d[tid] = 0.f;
-        sm_b[tid] = -3.0e38f; ss_b[tid] = 0.f;
-        szt_d[tid] = 0.f; szt_b[tid] = 0.f;
     }
     for (int e = tid; e < 4096; e += blockDim.x) sW2b[e] = a.w.W2[e];
     __syncthreads();
@@ -165,41 +170,51 @@ __global__ __launch_bounds__(256) void mlp_train_tile(MlpTrainArgs a)
     __syncthreads();
 
     const int ntile = noutp / 64;
-    // pass 1: row-wise log-sum-exp of both heads (online over column tiles)
-    for (int jt = 0; jt < ntile; ++jt) {
-        for (int e = tid; e < 4096; e += blockDim.x) {
-            const int h = e >> 6, c = e & 63;
-            sW3b[e] = a.w.W3[(size_t)h * noutp + jt * 64 + c];
-        }
-        __syncthreads();
-        f4v acc[2][2];
-        acc_zero(acc);
-        gemm64(sH2b, 64, 1, sW3b, 64, 1, 64, acc);
-        acc_each(acc, [&](int r, int c, float v) { sZ[r * 64 + c] = v + a.w.b3[jt * 64 + c]; });
-        __syncthreads();
-        if (tid < 64) {
-            const int r = tid;
-            for (int c = 0; c < 64; ++c) {
+    // pass 1: row-wise log-sum-exp of both heads, online over the column
+    // tiles; four threads per row, 16 columns each, merged at the end
+    {
+        const int row = tid >> 2, part = tid & 3;
+        const int td = std_[row], tb = stb_[row];
+        float md = -3.0e38f, sd = 0.f, mb = -3.0e38f, sb = 0.f, ztd = 0.f, ztb = 0.f;
+        for (int jt = 0; jt < ntile; ++jt) {
+            stage_w3_tile(a.w.W3, noutp, jt, sW3b);
+            __syncthreads();
+            f4v acc[2][2];
+            acc_zero(acc);
+            gemm64(sH2b, 64, 1, sW3b, 64, 1, 64, acc);
+            acc_each(acc, [&](int r, int c, float v) { sZ[r * 64 + c] = v + a.w.b3[jt * 64 + c]; });
+            __syncthreads();
+            for (int c = part * 16; c < part * 16 + 16; ++c) {
                 const int gc = jt * 64 + c;
-                const float z = sZ[r * 64 + c];
+                const float z = sZ[row * 64 + c];
                 if (gc < kMlpDir) {
-                    if (z > sm_d[r]) { ss_d[r] = ss_d[r] * expf(sm_d[r] - z) + 1.f; sm_d[r] = z; }
-                    else ss_d[r] += expf(z - sm_d[r]);
-                    if (gc == std_[r]) szt_d[r] = z;
+                    if (z > md) { sd = sd * expf(md - z) + 1.f; md = z; }
+                    else sd += expf(z - md);
+                    if (gc == td) ztd = z;
                 } else if (gc < kMlpDir + nbin) {
-                    if (z > sm_b[r]) { ss_b[r] = ss_b[r] * expf(sm_b[r] - z) + 1.f; sm_b[r] = z; }
-                    else ss_b[r] += expf(z - sm_b[r]);
-                    if (gc - kMlpDir == stb_[r]) szt_b[r] = z;
+                    if (z > mb) { sb = sb * expf(mb - z) + 1.f; mb = z; }
+                    else sb += expf(z - mb);
+                    if (gc - kMlpDir == tb) ztb = z;
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
-    }
-    if (tid < 64) {   // log-sum-exp per head; the sample's loss
-        const int r = tid;
-        sm_d[r] = sm_d[r] + logf(ss_d[r]);
-        sm_b[r] = sm_b[r] + logf(ss_b[r]);
-        sloss[r] = sw[r] * ((sm_d[r] - szt_d[r]) + (sm_b[r] - szt_b[r]));
+        for (int o = 1; o < 4; o <<= 1) {   // merge the four partial (max, sum) pairs of the row
+            const float md2 = __shfl_xor(md, o, 4), sd2 = __shfl_xor(sd, o, 4);
+            const float mb2 = __shfl_xor(mb, o, 4), sb2 = __shfl_xor(sb, o, 4);
+            const float mdn = fmaxf(md, md2), mbn = fmaxf(mb, mb2);
+            sd = sd * expf(md - mdn) + sd2 * expf(md2 - mdn);
+            sb = sb * expf(mb - mbn) + sb2 * expf(mb2 - mbn);
+            md = mdn;
+            mb = mbn;
+            ztd += __shfl_xor(ztd, o, 4);
+            ztb += __shfl_xor(ztb, o, 4);
+        }
+        if (part == 0) {   // log-sum-exp per head; the sample's loss
+            sm_d[row] = md + logf(sd);
+            sm_b[row] = mb + logf(sb);
+            sloss[row] = sw[row] * ((sm_d[row] - ztd) + (sm_b[row] - ztb));
+        }
     }
     __syncthreads();
 
@@ -207,10 +222,7 @@ __global__ __launch_bounds__(256) void mlp_train_tile(MlpTrainArgs a)
     f4v accH[2][2];
     acc_zero(accH);
     for (int jt = 0; jt < ntile; ++jt) {
-        for (int e = tid; e < 4096; e += blockDim.x) {
-            const int h = e >> 6, c = e & 63;
-            sW3b[e] = a.w.W3[(size_t)h * noutp + jt * 64 + c];
-        }
+        stage_w3_tile(a.w.W3, noutp, jt, sW3b);
         __syncthreads();
         {
             f4v acc[2][2];
@@ -519,14 +531,7 @@ __global__ __launch_bounds__(256) void mlp_proposal(GenArgs a, float* U)
         __syncthreads();
         float run = 0.f;                              // this row's running sum (held by all 4 lanes)
         for (int jt = 0; jt < noutp / 64; ++jt) {
-            {   // W3 tile: 64 rows of 128 contiguous bytes, 16 B per load
-                const uint4* src = (const uint4*)a.mlpw.W3;
-                uint4* dst = (uint4*)sW3b;
-                for (int e = tid; e < 512; e += blockDim.x) {
-                    const int h = e >> 3, c8 = e & 7;
-                    dst[e] = src[((size_t)h * noutp + jt * 64) / 8 + c8];
-                }
-            }
+            stage_w3_tile(a.mlpw.W3, noutp, jt, sW3b);
             __syncthreads();
             {
                 f4v acc[2][2];
