@@ -41,6 +41,14 @@ __device__ __forceinline__ s4v frag(const uint16_t* M, int rs, int cs, int r0, i
     const int r = r0 + (lane & 15), k = k0 + 4 * (lane >> 4);
     const uint16_t* p = M + r * rs + k * cs;
     s4v v;
+    if (cs == 1) {   // four consecutive BF16 (8-byte aligned): one LDS read
+        const uint2 u = *(const uint2*)p;
+        v.x = (short)(u.x & 0xFFFFu);
+        v.y = (short)(u.x >> 16);
+        v.z = (short)(u.y & 0xFFFFu);
+        v.w = (short)(u.y >> 16);
+        return v;
+    }
     v.x = (short)p[0];
     v.y = (short)p[cs];
     v.z = (short)p[2 * cs];
@@ -188,12 +196,12 @@ __global__ __launch_bounds__(256) void mlp_train_tile(MlpTrainArgs a)
                 const int gc = jt * 64 + c;
                 const float z = sZ[row * 64 + c];
                 if (gc < kMlpDir) {
-                    if (z > md) { sd = sd * expf(md - z) + 1.f; md = z; }
-                    else sd += expf(z - md);
+                    if (z > md) { sd = sd * __expf(md - z) + 1.f; md = z; }
+                    else sd += __expf(z - md);
                     if (gc == td) ztd = z;
                 } else if (gc < kMlpDir + nbin) {
-                    if (z > mb) { sb = sb * expf(mb - z) + 1.f; mb = z; }
-                    else sb += expf(z - mb);
+                    if (z > mb) { sb = sb * __expf(mb - z) + 1.f; mb = z; }
+                    else sb += __expf(z - mb);
                     if (gc - kMlpDir == tb) ztb = z;
                 }
             }
@@ -203,16 +211,16 @@ __global__ __launch_bounds__(256) void mlp_train_tile(MlpTrainArgs a)
             const float md2 = __shfl_xor(md, o, 4), sd2 = __shfl_xor(sd, o, 4);
             const float mb2 = __shfl_xor(mb, o, 4), sb2 = __shfl_xor(sb, o, 4);
             const float mdn = fmaxf(md, md2), mbn = fmaxf(mb, mb2);
-            sd = sd * expf(md - mdn) + sd2 * expf(md2 - mdn);
-            sb = sb * expf(mb - mbn) + sb2 * expf(mb2 - mbn);
+            sd = sd * __expf(md - mdn) + sd2 * __expf(md2 - mdn);
+            sb = sb * __expf(mb - mbn) + sb2 * __expf(mb2 - mbn);
             md = mdn;
             mb = mbn;
             ztd += __shfl_xor(ztd, o, 4);
             ztb += __shfl_xor(ztb, o, 4);
         }
         if (part == 0) {   // log-sum-exp per head; the sample's loss
-            sm_d[row] = md + logf(sd);
-            sm_b[row] = mb + logf(sb);
+            sm_d[row] = md + __logf(sd);
+            sm_b[row] = mb + __logf(sb);
             sloss[row] = sw[row] * ((sm_d[row] - ztd) + (sm_b[row] - ztb));
         }
     }
@@ -232,8 +240,8 @@ __global__ __launch_bounds__(256) void mlp_train_tile(MlpTrainArgs a)
                 const int gc = jt * 64 + c;
                 const float z = v + a.w.b3[gc];
                 float dz = 0.f;
-                if (gc < kMlpDir) dz = sw[r] * (expf(z - sm_d[r]) - (gc == std_[r] ? 1.f : 0.f));
-                else if (gc < kMlpDir + nbin) dz = sw[r] * (expf(z - sm_b[r]) - (gc - kMlpDir == stb_[r] ? 1.f : 0.f));
+                if (gc < kMlpDir) dz = sw[r] * (__expf(z - sm_d[r]) - (gc == std_[r] ? 1.f : 0.f));
+                else if (gc < kMlpDir + nbin) dz = sw[r] * (__expf(z - sm_b[r]) - (gc - kMlpDir == stb_[r] ? 1.f : 0.f));
                 sZ[r * 64 + c] = dz;
                 sdZb[r * 64 + c] = f2bf(dz);
             });
