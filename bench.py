@@ -8,9 +8,13 @@ temperature snapshots (profile + Fourier modes) in one launch, synthetic
 100x16x69 correlated-k table, 100 rays/cell (2.0e10 rays per step).
 
 Prints ONE JSON line with rays/s (whole job), the traversal kernel's
-algorithmic HBM bandwidth against the MI355X roofline, and the CPU baseline
-(the oracle's restatement of the reference algorithm on this host's cores,
-timed on a bounded sample of emitting cells of the same workload).
+algorithmic HBM bandwidth against the MI355X roofline (with the HBM traffic
+and L2 hit rate measured by rocprof for the same command, profiles/), the
+CPU baseline (the oracle's restatement of the reference algorithm on this
+host's cores, timed on a bounded sample of emitting cells), and the learned
+proposal against the plain estimator: RMS / max error of Q_rad against a
+converged FP64 reference and the time to 1% RMS, on the fine grid and with
+the reference's default mesh coarsening.
 """
 import argparse
 import json
@@ -226,7 +230,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(emitters shard by construction; see the sharding notes in DESIGN.md)", file=sys.stderr)
+              "(emitting cells are independent units of work: a multi-GPU solve shards them, replicating the field)", file=sys.stderr)
         sys.exit(2)
 
     import dnsmc_amd
