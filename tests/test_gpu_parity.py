@@ -401,3 +401,36 @@ def test_fp64_traversal_with_coarsening_matches_fp64_oracle(oracle, ck_table_dir
     np.testing.assert_allclose(res["E"], E, rtol=1e-12, atol=0)
     np.testing.assert_allclose(res["A"], A, rtol=1e-12, atol=0)
     assert st["n_steps"] == ns
+
+
+def test_reference_driver_resfortm():
+    """The reference's only checked-in result of the hot path, code/resfortm.txt:
+    its standalone driver (code/rad_main.f) with the shipped configuration
+    (code/param.h: 64^3, particle tables, 5 coarsening levels, photmax 20000,
+    nVar 3, Sobol directions; T = 500 - 2000 x^2 + 2000 x, code/rad_main.f:68),
+    plane means over (j, k) of Q, of its standard error and the spatial RMS.
+    Our compat mode uses its own XORWOW seeding, so the agreement is
+    statistical: each plane mean within 3 per-cell standard errors (the Sobol
+    directions are shared by all cells, so plane means are not averaged down),
+    and the error estimates themselves within 10 % / 15 %."""
+    from helpers import read_resfortm
+    cfg = dnsmc_amd.config_from_dict(dict(table_dir=particle_table_dir()), dnsmc_amd.default_config())
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    x = S.array("centres0", 0)[1:-1].astype(np.float64)
+    ref = read_resfortm()
+    np.testing.assert_allclose(x, ref[:, 0], rtol=0, atol=8e-8)
+    Tx = 500.0 - 2000.0 * x ** 2 + 2000.0 * x
+    T = np.broadcast_to(Tx[:, None, None], tuple(cfg.n)).astype(np.float32)
+    S.set_field(0, workloads.ghosted(np.ascontiguousarray(T)))
+    S.run()
+    r = S.results(0)
+    n = tuple(cfg.n)
+    Q = r["Q"].reshape(n[2], n[1], n[0])
+    Qm = Q.mean(axis=(0, 1))
+    Vm = r["sdev"].reshape(n[2], n[1], n[0]).mean(axis=(0, 1))
+    rms = Q.std(axis=(0, 1))
+    assert np.all(np.abs(Qm - ref[:, 1]) <= 3.0 * ref[:, 2]), np.max(np.abs(Qm - ref[:, 1]) / ref[:, 2])
+    assert np.mean(np.abs(Qm[2:-2] / ref[2:-2, 1] - 1.0)) < 5e-4
+    np.testing.assert_allclose(Vm, ref[:, 2], rtol=0.10)
+    np.testing.assert_allclose(rms, ref[:, 3], rtol=0.15)
