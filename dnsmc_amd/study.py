@@ -9,6 +9,8 @@ tests check exactly that), on the GPU's throughput.
   python -m dnsmc_amd.study errors       RMS / max error of Q_rad per workload
   python -m dnsmc_amd.study precision    FP32 vs FP64 traversal x FP64 vs
                                          Kahan-FP32 tallies, 256x128x128
+  python -m dnsmc_amd.study heldout      learned proposal on held-out 64x96x64
+                                         snapshots: variance ratio, errors
   python -m dnsmc_amd.study convergence  error vs rays per cell (R^-1/2)
 
 Errors over the chosen cells: eps_rms = sqrt(sum (Q - Q_ref)^2 - sum var_ref)
@@ -180,12 +182,45 @@ def study_convergence(workload="ck384x8", rays=(100, 200, 400, 800), ncells=1024
     return dict(workload=workload, snapshot=0, cells=len(cells), reference_rays_per_cell=rays_ref, legs=res)
 
 
+def study_heldout(rays=(100, 1000), snaps=range(51, 56), **cfg_kw):
+    """The learned proposal on held-out channel snapshots (64x96x64, parabolic
+    profile + N(0, 10 K), seeds 51-55; trained on seeds 31-50): variance
+    ratio vs the plain estimator at equal rays per cell (sums of the per-cell
+    variances), and both estimators' RMS error against a converged FP64
+    reference on 1024 cells of each snapshot."""
+    from .learned import train_channel_proposal
+    W, info = train_channel_proposal(**cfg_kw)
+    wl = "ck64"
+    cells = sample_cells(wl, 1024)
+    out = dict(train=info, snapshots={})
+    for seed in snaps:
+        fields = [workloads.field_profile_noise(seed, workloads.WORKLOADS[wl]["n"])]
+        Qref, vref, _ = reference(wl, cells, 65536, fields=fields, **cfg_kw)
+        row = {}
+        for R in rays:
+            res = {}
+            for name, mode, Wm in (("plain", 0, None), ("learned", 2, W)):
+                S, _ = make_solver(wl, R, mode=mode, mlp=Wm, mix_eps=0.05, fields=fields, **cfg_kw)
+                S.run()
+                res[name] = S.results(0)
+                S.close()
+            vr = float(np.sum(res["plain"]["sdev"] ** 2) / np.sum(res["learned"]["sdev"] ** 2))
+            row[str(R)] = dict(variance_ratio=vr,
+                               rms_plain=errors(res["plain"]["Q"][cells], Qref, vref)["rms"],
+                               rms_learned=errors(res["learned"]["Q"][cells], Qref, vref)["rms"])
+            log("snapshot", seed, "R", R, row[str(R)])
+        out["snapshots"][str(seed)] = row
+    return out
+
+
 def main(argv):
     what = argv[1] if len(argv) > 1 else "errors"
     if what == "errors":
         out = study_errors()
     elif what == "precision":
         out = study_precision()
+    elif what == "heldout":
+        out = study_heldout()
     elif what == "convergence":
         mlp = None
         if "--learned" in argv:
@@ -193,7 +228,7 @@ def main(argv):
             mlp, _ = train_channel_proposal()
         out = study_convergence(mlp=mlp)
     else:
-        raise SystemExit("usage: python -m dnsmc_amd.study errors|precision|convergence [--learned]")
+        raise SystemExit("usage: python -m dnsmc_amd.study errors|precision|heldout|convergence [--learned]")
     print(json.dumps(out), flush=True)
 
 
