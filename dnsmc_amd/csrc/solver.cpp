@@ -169,6 +169,7 @@ struct dnsmc_solver {
     static constexpr int kPipes = 3;
     Pipe pipe[kPipes];
     hipEvent_t ev_ready = nullptr;
+    hipEvent_t ev_gen[2] = {nullptr, nullptr}, ev_tr[2] = {nullptr, nullptr};   // snapshot-wise learned path
     // tallies and results
     DevBuf<double> d_E, d_A, d_Q2, d_Q, d_Eo, d_Ao, d_sd;
     DevBuf<unsigned long long> d_nsteps;
@@ -629,8 +630,24 @@ static int do_run(dnsmc_solver* s)
                  nslots * (long long)R <= (1ll << 32) - (1ll << 20);
     if (const char* fe = getenv("DNSMC_FUSED")) fused = fused && atoi(fe) != 0;
     if (fused) slots_per_batch = nslots;
-    const long long batch_rays = fused ? 0 : std::min(slots_per_batch, nslots) * R;
-    const long long nbatch_total = fused ? 1 : ((nslots + slots_per_batch - 1) / slots_per_batch) * c.nsnap;
+    // learned estimator, atomic tallies, no records: the descriptors of a whole
+    // snapshot are generated (batch by batch, on a second stream) before one
+    // traversal launch covers them, and the next snapshot is generated while
+    // this one is traced (two snapshot-sized descriptor sets; DNSMC_SNAPWISE=0:
+    // batched)
+    bool snapwise = c.mode == DNSMC_MODE_LEARNED && s->have_mlp && c.precision == 0 &&
+                    c.tally == DNSMC_TALLY_F64_ATOMIC && !s->record && nslots * (long long)R <= (1ll << 32) - (1ll << 20);
+    if (const char* se = getenv("DNSMC_SNAPWISE")) snapwise = snapwise && atoi(se) != 0;
+    if (snapwise) {
+        size_t fr = 0, tot = 0;
+        const size_t need = 2 * (size_t)(nslots * R) * 12 + (size_t)std::min(slots_per_batch, nslots) *
+                                                               mlp_noutp(s->mlp_nbin) * 4;
+        const size_t have = (size_t)s->pipe[0].d_w0.n * 12 + (size_t)s->pipe[1].d_w0.n * 12 +
+                            (size_t)s->pipe[1].d_qcdf.n * 4;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess || need + ((size_t)4 << 30) > fr + have) snapwise = false;
+    }
+    const long long batch_rays = fused ? 0 : (snapwise ? nslots * R : std::min(slots_per_batch, nslots) * R);
+    const long long nbatch_total = fused ? 1 : (snapwise ? 2 : ((nslots + slots_per_batch - 1) / slots_per_batch) * c.nsnap);
     int pipes_max = dnsmc_solver::kPipes;   // DNSMC_PIPES=1 serialises the batches (standalone kernel timing)
     if (const char* pe = getenv("DNSMC_PIPES")) pipes_max = std::max(1, std::min(dnsmc_solver::kPipes, atoi(pe)));
     const int npipes = (int)std::min<long long>(pipes_max, nbatch_total);
@@ -639,10 +656,10 @@ static int do_run(dnsmc_solver* s)
         HIPCHK(pp.d_w0.ensure(batch_rays));
         HIPCHK(pp.d_w1.ensure(batch_rays));
         HIPCHK(pp.d_w.ensure(batch_rays));
-        HIPCHK(pp.d_winv.ensure(batch_rays));
+        HIPCHK(pp.d_winv.ensure(snapwise ? 0 : batch_rays));
         if (det && !f64) HIPCHK(pp.d_pray.ensure(batch_rays));
         if (f64) HIPCHK(pp.d_pray64.ensure(batch_rays));
-        if (c.mode == DNSMC_MODE_LEARNED && s->have_mlp)
+        if (c.mode == DNSMC_MODE_LEARNED && s->have_mlp && (!snapwise || p == 1))
             HIPCHK(pp.d_qcdf.ensure((size_t)std::min(slots_per_batch, nslots) * mlp_noutp(s->mlp_nbin)));
         HIPCHK(pp.d_claim.ensure(256));
     }
@@ -765,11 +782,11 @@ static int do_run(dnsmc_solver* s)
         // measured defaults: the fused launch refills at 24 idle lanes and
         // hands out 512-ray super-chunks; the batched launches 16 and 256
         const char* rk = getenv("DNSMC_REFILL_K");
-        P.refill_k = rk ? atoi(rk) : (fused ? 24 : 16);
+        P.refill_k = rk ? atoi(rk) : (fused || snapwise ? 24 : 16);
         if (P.refill_k < 1) P.refill_k = 1;
         if (P.refill_k > 64) P.refill_k = 64;
         const char* sc = getenv("DNSMC_SUPER");
-        P.super = sc ? atoi(sc) : (fused ? 512 : 256);
+        P.super = sc ? atoi(sc) : (fused || snapwise ? 512 : 256);
         P.super = std::max(64, (P.super + 63) / 64 * 64);
     }
     P.slotinfo = s->d_slotinfo.p;
@@ -870,8 +887,60 @@ static int do_run(dnsmc_solver* s)
         HIPCHK(hipEventRecord(et.b, st));
         s->st.n_trace_launches++;
     }
+    if (snapwise) {
+        // generation on pipe 1 into descriptor set (snap % 2), one traversal
+        // launch per snapshot on the main stream
+        hipStream_t gs = s->pipe[1].stream, ts = s->pipe[0].stream;
+        const size_t lds_prop = sizeof(float) * (64 * kMlpIn + 4096 + 64) + 4 * 4096 * sizeof(uint16_t);
+        for (int snap = 0; snap < c.nsnap; ++snap) {
+            const int b = snap & 1;
+            dnsmc_solver::Pipe& buf = s->pipe[b];
+            if (snap >= 2) HIPCHK(hipStreamWaitEvent(gs, s->ev_tr[b], 0));   // trace(snap - 2) done with the set
+            EventPair& eg = next_event(s, 0);
+            HIPCHK(hipEventRecord(eg.a, gs));
+            for (long long b0 = 0; b0 < nslots; b0 += slots_per_batch) {
+                const long long nsl = std::min(slots_per_batch, nslots - b0);
+                G.slot0 = b0;
+                G.nslots = nsl;
+                G.snap = snap;
+                G.w0 = buf.d_w0.p + b0 * R;
+                G.w1 = buf.d_w1.p + b0 * R;
+                G.w = buf.d_w.p + b0 * R;
+                G.winv = nullptr;
+                int nb = (int)std::min<long long>((nsl + 63) / 64, (long long)s->ncu * 3);
+                hipLaunchKernelGGL(mlp_proposal, dim3(nb), dim3(256), lds_prop, gs, G, s->pipe[1].d_qcdf.p);
+                nb = (int)std::min<long long>((nsl + 7) / 8, (long long)s->ncu * 3);
+                hipLaunchKernelGGL(raygen_learned, dim3(nb), dim3(256), lds_learn, gs, G,
+                                   (const float*)s->pipe[1].d_qcdf.p, (const float*)s->d_pcdf.p);
+                HIPCHK(hipGetLastError());
+            }
+            HIPCHK(hipEventRecord(eg.b, gs));
+            HIPCHK(hipEventRecord(s->ev_gen[b], gs));
+            HIPCHK(hipStreamWaitEvent(ts, s->ev_gen[b], 0));
+            P.w0 = buf.d_w0.p; P.w1 = buf.d_w1.p; P.w = buf.d_w.p; P.winv = nullptr;
+            P.pray = nullptr;
+            P.nrays = nslots * R;
+            P.slot0 = 0;
+            P.snap = snap;
+            P.E = s->d_E.p + (size_t)snap * nslots;
+            P.A = s->d_A.p + (size_t)snap * nslots;
+            P.Q2 = s->d_Q2.p + (size_t)snap * nslots;
+            P.nrec = 0;
+            P.claim = s->pipe[0].d_claim.p;
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, ts, s->pipe[0].d_claim.p, 256ll, 0u);
+            HIPCHK(hipGetLastError());
+            const long long nsc = (P.nrays + P.super - 1) / P.super;
+            const int tb_cap = (int)std::min<long long>(trace_blocks, std::max<long long>(8, (nsc + 15) / 16));
+            EventPair& et = next_event(s, 1);
+            HIPCHK(hipEventRecord(et.a, ts));
+            HIPCHK(launch_trace(spec, tab, false, false, false, amc, s->ncu, tb_cap, lds_trace, ts, P));
+            HIPCHK(hipEventRecord(et.b, ts));
+            HIPCHK(hipEventRecord(s->ev_tr[b], ts));
+            s->st.n_trace_launches++;
+        }
+    }
     long long bi = 0;
-    for (int snap = 0; snap < c.nsnap && !fused; ++snap) {
+    for (int snap = 0; snap < c.nsnap && !fused && !snapwise; ++snap) {
         for (long long b0 = 0; b0 < nslots; b0 += slots_per_batch, ++bi) {
             dnsmc_solver::Pipe& pp = s->pipe[bi % npipes];
             hipStream_t st = pp.stream;
@@ -1061,6 +1130,12 @@ int dnsmc_create(const dnsmc_config* cfg, dnsmc_solver** out)
     for (int p = 0; p < dnsmc_solver::kPipes; ++p)
         if (hipEventCreateWithFlags(&s->pipe[p].done, hipEventDisableTiming) != hipSuccess) { dnsmc_destroy(s); return fail(DNSMC_EDEVICE, "hipEventCreate failed"); }
     if (hipEventCreateWithFlags(&s->ev_ready, hipEventDisableTiming) != hipSuccess) { dnsmc_destroy(s); return fail(DNSMC_EDEVICE, "hipEventCreate failed"); }
+    for (int b = 0; b < 2; ++b)
+        if (hipEventCreateWithFlags(&s->ev_gen[b], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_tr[b], hipEventDisableTiming) != hipSuccess) {
+            dnsmc_destroy(s);
+            return fail(DNSMC_EDEVICE, "hipEventCreate failed");
+        }
     s->Tmax_snap.assign(cfg->nsnap, 0.f);
     s->Tlo_snap.assign(cfg->nsnap, 0.f);
     s->Thi_snap.assign(cfg->nsnap, 0.f);
@@ -1093,6 +1168,10 @@ void dnsmc_destroy(dnsmc_solver* s)
     for (int p = 0; p < dnsmc_solver::kPipes; ++p)
         if (s->pipe[p].done) (void)hipEventDestroy(s->pipe[p].done);
     if (s->ev_ready) (void)hipEventDestroy(s->ev_ready);
+    for (int b = 0; b < 2; ++b) {
+        if (s->ev_gen[b]) (void)hipEventDestroy(s->ev_gen[b]);
+        if (s->ev_tr[b]) (void)hipEventDestroy(s->ev_tr[b]);
+    }
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
