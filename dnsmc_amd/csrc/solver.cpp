@@ -640,7 +640,7 @@ static int do_run(dnsmc_solver* s)
     if (const char* se = getenv("DNSMC_SNAPWISE")) snapwise = snapwise && atoi(se) != 0;
     if (snapwise) {
         size_t fr = 0, tot = 0;
-        const size_t need = 2 * (size_t)(nslots * R) * 12 + (size_t)std::min(slots_per_batch, nslots) *
+        const size_t need = (c.nsnap < 2 ? 1 : 2) * (size_t)(nslots * R) * 12 + (size_t)std::min(slots_per_batch, nslots) *
                                                                mlp_noutp(s->mlp_nbin) * 4;
         const size_t have = (size_t)s->pipe[0].d_w0.n * 12 + (size_t)s->pipe[1].d_w0.n * 12 +
                             (size_t)s->pipe[1].d_qcdf.n * 4;
@@ -650,12 +650,15 @@ static int do_run(dnsmc_solver* s)
     const long long nbatch_total = fused ? 1 : (snapwise ? 2 : ((nslots + slots_per_batch - 1) / slots_per_batch) * c.nsnap);
     int pipes_max = dnsmc_solver::kPipes;   // DNSMC_PIPES=1 serialises the batches (standalone kernel timing)
     if (const char* pe = getenv("DNSMC_PIPES")) pipes_max = std::max(1, std::min(dnsmc_solver::kPipes, atoi(pe)));
-    const int npipes = (int)std::min<long long>(pipes_max, nbatch_total);
+    // the snapshot-wise path always uses streams 0 (traversal) and 1 (generation)
+    const int npipes = snapwise ? 2 : (int)std::min<long long>(pipes_max, nbatch_total);
     for (int p = 0; p < npipes; ++p) {
         dnsmc_solver::Pipe& pp = s->pipe[p];
-        HIPCHK(pp.d_w0.ensure(batch_rays));
-        HIPCHK(pp.d_w1.ensure(batch_rays));
-        HIPCHK(pp.d_w.ensure(batch_rays));
+        // one snapshot needs only descriptor set 0
+        const long long nset = snapwise && p == 1 && c.nsnap < 2 ? 0 : batch_rays;
+        HIPCHK(pp.d_w0.ensure(nset));
+        HIPCHK(pp.d_w1.ensure(nset));
+        HIPCHK(pp.d_w.ensure(nset));
         HIPCHK(pp.d_winv.ensure(snapwise ? 0 : batch_rays));
         if (det && !f64) HIPCHK(pp.d_pray.ensure(batch_rays));
         if (f64) HIPCHK(pp.d_pray64.ensure(batch_rays));

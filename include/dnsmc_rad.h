@@ -165,11 +165,16 @@ int dnsmc_set_emitters_list(dnsmc_solver *s, const int *cells, int ncells);
 
 /* Queue the whole radiation solve (all snapshots) and return.
  * The plain estimator with FP64 atomic tallies and records off draws its
- * rays inside the traversal, one launch for all snapshots; the other modes
- * generate descriptors in batches of batch_rays on three pipeline streams.
+ * rays inside the traversal, one launch for all snapshots; the learned
+ * estimator with FP64 atomic tallies and records off generates the
+ * descriptors of a whole snapshot on a second stream while the previous
+ * snapshot is traced (one traversal launch per snapshot, two snapshot-sized
+ * descriptor sets when free device memory allows); the other modes generate
+ * descriptors in batches of batch_rays on three pipeline streams.
  * Environment overrides, read at each run (measurement and diagnosis; the
  * defaults are the measured best):
  *   DNSMC_FUSED=0     batched generation for the plain estimator too
+ *   DNSMC_SNAPWISE=0  batched generation for the learned estimator too
  *   DNSMC_PIPES=1..3  pipeline streams of the batched modes
  *   DNSMC_TABLES=global  keep kappa / Planck tables out of LDS
  *   DNSMC_REFILL_K=k  idle lanes of a wave before it starts new rays

@@ -215,29 +215,34 @@ def test_learned_estimator_unbiased_with_coarsening(trained):
     assert np.sum(plain["sdev"] ** 2) / np.sum(lrn["sdev"] ** 2) > 2.0
 
 
-def test_snapshotwise_learned_matches_batched(trained, monkeypatch):
-    """Learned solve of three snapshots: descriptors of a whole snapshot and
-    one traversal launch per snapshot (generation of the next overlapped)
-    against the batched pipelines: the same rays, so equal step totals and
-    tallies equal up to the order of the FP64 atomic additions."""
+@pytest.mark.parametrize("nsnap,pipes", [(3, None), (1, None), (3, "1")])
+def test_snapshotwise_learned_matches_batched(trained, monkeypatch, nsnap, pipes):
+    """Learned solve of one or three snapshots: descriptors of a whole
+    snapshot and one traversal launch per snapshot (generation of the next
+    overlapped) against the batched pipelines: the same rays, so equal step
+    totals and tallies equal up to the order of the FP64 atomic additions.
+    DNSMC_PIPES=1 restricts the batched loop only; the snapshot-wise path
+    keeps its two streams."""
     net, _, tdir, n = trained
+    if pipes is not None:
+        monkeypatch.setenv("DNSMC_PIPES", pipes)
     out = {}
     for sw in ("1", "0"):
         monkeypatch.setenv("DNSMC_SNAPWISE", sw)
-        cfg = workloads.channel_config(n, rays_per_cell=64, spectral=0, Tw=573.0, table_dir=tdir, mode=2, nsnap=3,
+        cfg = workloads.channel_config(n, rays_per_cell=64, spectral=0, Tw=573.0, table_dir=tdir, mode=2, nsnap=nsnap,
                                        batch_rays=64 * 1000)
         S = dnsmc_amd.Solver(cfg)
         S.load_tables()
         S.set_mlp(*net.host_weights(), mix_eps=0.1)
-        for s in range(3):
+        for s in range(nsnap):
             S.set_field(s, _field(70 + s, n))
         S.run()
         st = S.stats()
-        out[sw] = (st, [S.results(s) for s in range(3)])
+        out[sw] = (st, [S.results(s) for s in range(nsnap)])
         S.close()
     (s1, r1), (s0, r0) = out["1"], out["0"]
-    assert s1["n_trace_launches"] == 3 and s0["n_trace_launches"] > 3
+    assert s1["n_trace_launches"] == nsnap and s0["n_trace_launches"] > nsnap
     assert s1["n_steps"] == s0["n_steps"]
-    for s in range(3):
+    for s in range(nsnap):
         np.testing.assert_allclose(r1[s]["E"], r0[s]["E"], rtol=1e-12, atol=0)
         np.testing.assert_allclose(r1[s]["A"], r0[s]["A"], rtol=1e-12, atol=0)
