@@ -414,6 +414,34 @@ __device__ __forceinline__ void tally_ray(const TraceArgs& P, float er, float ar
     aQ += x * x;
 }
 
+// A drawn ray, as the refill hands it to a lane: bricked index and packed
+// (i, j, k) of the emitting cell, its tally slot, band | g << 16 | sign flags
+// << 24, emitter temperature, direction and its reciprocals, weights.
+struct RayStart {
+    int idx, cell, six, bg;
+    float Tc, sx, sy, sz, ix, iy, iz, rw, rwi;
+};
+
+// Lane `src`'s ray (all 64 lanes call it).
+__device__ __forceinline__ RayStart pull_ray(const RayStart& r, int src)
+{
+    RayStart o;
+    o.idx = __shfl(r.idx, src);
+    o.cell = __shfl(r.cell, src);
+    o.six = __shfl(r.six, src);
+    o.bg = __shfl(r.bg, src);
+    o.Tc = __shfl(r.Tc, src);
+    o.sx = __shfl(r.sx, src);
+    o.sy = __shfl(r.sy, src);
+    o.sz = __shfl(r.sz, src);
+    o.ix = __shfl(r.ix, src);
+    o.iy = __shfl(r.iy, src);
+    o.iz = __shfl(r.iz, src);
+    o.rw = __shfl(r.rw, src);
+    o.rwi = __shfl(r.rwi, src);
+    return o;
+}
+
 // Persistent cell-by-cell traversal (code/mc_gpu.cu:228-438).
 //
 // Work: the launch's rays (in 3-D brick order of their emitting cells) are
@@ -542,13 +570,110 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     gfloat* Tl = lev_T(s_lev);
     gfloat* T0 = Tl;
     uint32_t rsnap = 0;
+    // GEN: the wave's queue of drawn rays (lane j: ray gbase + j, j < gn)
+    uint32_t gn = 0, ghead = 0, gsnap = 0;
+    RayStart Qr = {};
+
+    // ray q of snapshot qs of the launch: drawn here (GEN, the plain estimator's
+    // Philox draw, bit-identical to raygen_plain) or read from its descriptors
+    auto draw_ray = [&](uint32_t q, uint32_t qs) -> RayStart {
+        RayStart rd;
+        const uint32_t slot = fdiv(q, P.fd_R);
+        const int4 si = P.slotinfo[P.slot0 + slot];
+        rd.six = si.x;
+        rd.idx = si.z;
+        rd.cell = si.w;
+        gfloat* Te = lev_T(s_lev) + (GEN ? (size_t)qs * (uint32_t)s_lev[14] : (size_t)0);
+        const float Tq = Te[si.z];
+        rd.Tc = Tq;
+        int bnd, g;
+        uint32_t mt, mp;
+        float ratio = 1.0f;       // p/q of a learned descriptor
+        bool weigh = GEN;         // emission weight computed here (tables in LDS)
+        if (GEN) {
+            const uint32_t r = q - slot * (uint32_t)P.rays_per_slot;
+            const u32x4 x = philox4x32_10(u32x4{r, (uint32_t)si.y, (uint32_t)(P.snap + (int)qs), 0u},
+                                          (uint32_t)P.seed, (uint32_t)(P.seed >> 32));
+            bnd = cdf_search(s_cdfb, P.nB, u01_open(x.x));
+            if (!P.g_search)
+                g = 0;
+            else if (P.gen_lds)
+                g = cdf_search(s_cdfg + bnd * P.nQ, P.nQ, u01_open(x.y));
+            else
+                g = cdf_search(P.cdf_g + bnd * P.nQ, P.nQ, u01_open(x.y));
+            g = g > P.nQ - 1 ? P.nQ - 1 : g;
+            mt = x.z >> 8;
+            mp = x.w >> 8;
+        } else {
+            const uint32_t a0 = P.w0[q], a1 = P.w1[q];
+            bnd = desc_band(a0, a1);
+            g = desc_g(a1);
+            mt = a0 & 0xFFFFFFu;
+            mp = a1 & 0xFFFFFFu;
+            if (P.desc_ratio) {
+                ratio = P.w[q];
+                weigh = true;
+            } else {
+                rd.rw = P.w[q];
+                rd.rwi = P.winv[q];
+            }
+        }
+        if (weigh) {
+            // Ibmax (k_i Ib_i) / (k_max Ib_max) (code/mc_gpu.cu:63, :163-169),
+            // the arithmetic of the generators' bundle weights
+            float kap, ib;
+            if (SPEC == kSpecGreyT4) {
+                const float t2 = Tq * Tq;
+                kap = P.kappa_grey;
+                ib = t2 * t2;
+            } else {
+                float pos = fmaf(Tq, P.inv_dT, P.mT0);
+                pos = fminf(fmaxf(pos, 0.0f), (float)(nT - 1));
+                int t0 = (int)pos;
+                t0 = t0 > nT - 2 ? nT - 2 : t0;
+                const float fr = pos - (float)t0;
+                const int kr = (bnd * P.nQ + g) * rowlen, ir = bnd * rowlen;
+                const int t = TAB == kTabSliced ? max(0, min(t0 - P.tab_lo, nw - 2)) : t0;
+                const float k0 = TAB == kTabSliced ? s_kq[kr + t] : kq[kr + t];
+                const float k1 = TAB == kTabSliced ? s_kq[kr + t + 1] : kq[kr + t + 1];
+                const float i0 = ibt[ir + t], i1 = ibt[ir + t + 1];
+                kap = fmaf(fr, k1 - k0, k0);
+                ib = fmaf(fr, i1 - i0, i0);
+            }
+            const float iem = P.gen_lds ? s_iem[bnd * P.nQ + g] : P.inv_emax[bnd * P.nQ + g];
+            const float w = P.Ibmax * (kap * ib) * iem;
+            const float wi = w * (1.0f / ib);
+            rd.rw = GEN ? w : w * ratio;
+            rd.rwi = GEN ? wi : wi * ratio;
+        }
+        const float ut = ((float)mt + P.u_off) * kTwoM24;
+        const float up = ((float)mp + P.u_off) * kTwoM24;
+        const float cth = 1.0f - 2.0f * ut;
+        const float sth = 2.0f * sqrtf(ut * (1.0f - ut));
+        float sp, cp;
+        sincos2pi(up, &sp, &cp);
+        const float s1 = sth * cp, s2 = sth * sp;
+        float dx = P.polar == 0 ? cth : (P.polar == 1 ? s2 : s1);
+        float dy = P.polar == 0 ? s1 : (P.polar == 1 ? cth : s2);
+        float dz = P.polar == 0 ? s2 : (P.polar == 1 ? s1 : cth);
+        if (dx == 0.0f) dx = 1e-10f;
+        if (dy == 0.0f) dy = 1e-10f;
+        if (dz == 0.0f) dz = 1e-10f;
+        const int f = (dx < 0.0f ? 1 : 0) | (dy < 0.0f ? 2 : 0) | (dz < 0.0f ? 4 : 0);
+        rd.sx = dx; rd.sy = dy; rd.sz = dz;
+        rd.ix = 1.0f / dx;
+        rd.iy = 1.0f / dy;
+        rd.iz = 1.0f / dz;
+        rd.bg = bnd | (g << 16) | (f << 24);
+        return rd;
+    };
 
     for (;;) {
         const bool idle = !alive;
         const unsigned long long m_idle = __ballot(idle);
         const int ni = __popcll(m_idle);
         const int na = 64 - ni;
-        const bool have_rays = more || cur < cend;
+        const bool have_rays = more || cur < cend || ghead < gn;
         if (na == 0 && !have_rays) break;
         if (have_rays && (ni >= P.refill_k || na == 0)) {
             // ---- tally the rays finished since the last refill
@@ -565,141 +690,23 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 steps_lane += (uint32_t)rsteps;
                 fin = false;
             }
-            // ---- hand out consecutive rays to the idle lanes, in lane order
             const uint32_t rank = (uint32_t)__popcll(m_idle & lanemask_lt);
-            const uint32_t avail = cend - cur;
-            uint32_t q = 0xFFFFFFFFu, qs = csnap;
-            if (idle && rank < avail) q = cur + rank;
-            if (avail < (uint32_t)ni && more) {
-                // the current super-chunk runs out: claim the next one, from this
-                // XCD's share first, then from the following XCDs' shares.  Claim
-                // c of XCD X is chunk X*per_x + c % per_x of snapshot c / per_x.
-                bool got = false;
-                uint32_t nstart = 0, nend = 0, nsn = 0;
-                while (xi < nparts) {
-                    const uint32_t X = (xcd + xi) % nparts;
-                    uint32_t c = 0;
-                    if (lane == 0) c = atomicAdd(P.claim + 32u * X, 1u);
-                    c = __builtin_amdgcn_readfirstlane(c);
-                    if (c >= nclaim) {
-                        ++xi;
-                        continue;
-                    }
-                    const uint32_t sn = c / per_x;
-                    const uint32_t lc = X * per_x + (c - sn * per_x);
-                    if (lc >= cps) continue;   // past the snapshot's last chunk
-                    nsn = sn;
-                    nstart = lc * S;
-                    nend = nstart + S < nrays ? nstart + S : nrays;
-                    got = true;
-                    break;
-                }
-                if (got) {
-                    if (idle && rank >= avail && nstart + (rank - avail) < nend) {
-                        q = nstart + (rank - avail);
-                        qs = nsn;
-                    }
-                    const uint32_t used = (uint32_t)ni - avail;
-                    cur = nstart + used < nend ? nstart + used : nend;
-                    cend = nend;
-                    csnap = nsn;
-                } else {
-                    more = false;
-                    cur = cend;
-                }
-            } else {
-                cur += avail < (uint32_t)ni ? avail : (uint32_t)ni;
-            }
-            // ---- start the handed-out rays
-            if (q != 0xFFFFFFFFu) {
-                qcur = q;
-                rsnap = qs;
-                const uint32_t slot = fdiv(q, P.fd_R);
-                const int4 si = P.slotinfo[P.slot0 + slot];
-                rslot = si.x + (int)rsnap * P.tally_stride;
+            // the drawn ray rd of snapshot rsn becomes this lane's ray
+            auto start_ray = [&](const RayStart& rd, uint32_t rsn) {
+                rsnap = rsn;
+                rslot = rd.six + (int)rsnap * P.tally_stride;
                 if (GEN) T0 = lev_T(s_lev) + (size_t)rsnap * (uint32_t)s_lev[14];
-                Tc = T0[si.z];
-                int g;
-                uint32_t mt, mp;
-                float ratio = 1.0f;       // p/q of a learned descriptor
-                bool weigh = GEN;         // emission weight computed here (tables in LDS)
-                if (GEN) {
-                    // the plain draw of ray r of the cell (raygen_plain)
-                    const uint32_t r = q - slot * (uint32_t)P.rays_per_slot;
-                    const u32x4 x = philox4x32_10(u32x4{r, (uint32_t)si.y, (uint32_t)(P.snap + (int)rsnap), 0u},
-                                                  (uint32_t)P.seed, (uint32_t)(P.seed >> 32));
-                    band = cdf_search(s_cdfb, P.nB, u01_open(x.x));
-                    if (!P.g_search)
-                        g = 0;
-                    else if (P.gen_lds)
-                        g = cdf_search(s_cdfg + band * P.nQ, P.nQ, u01_open(x.y));
-                    else
-                        g = cdf_search(P.cdf_g + band * P.nQ, P.nQ, u01_open(x.y));
-                    g = g > P.nQ - 1 ? P.nQ - 1 : g;
-                    mt = x.z >> 8;
-                    mp = x.w >> 8;
-                } else {
-                    const uint32_t a0 = P.w0[q], a1 = P.w1[q];
-                    band = desc_band(a0, a1);
-                    g = desc_g(a1);
-                    mt = a0 & 0xFFFFFFu;
-                    mp = a1 & 0xFFFFFFu;
-                    if (P.desc_ratio) {
-                        ratio = P.w[q];
-                        weigh = true;
-                    } else {
-                        rw = P.w[q];
-                        rwi = P.winv[q];
-                    }
-                }
-                if (weigh) {
-                    // Ibmax (k_i Ib_i) / (k_max Ib_max) (code/mc_gpu.cu:63, :163-169),
-                    // the arithmetic of the generators' bundle weights
-                    float kap, ib;
-                    if (SPEC == kSpecGreyT4) {
-                        const float t2 = Tc * Tc;
-                        kap = P.kappa_grey;
-                        ib = t2 * t2;
-                    } else {
-                        float pos = fmaf(Tc, P.inv_dT, P.mT0);
-                        pos = fminf(fmaxf(pos, 0.0f), (float)(nT - 1));
-                        int t0 = (int)pos;
-                        t0 = t0 > nT - 2 ? nT - 2 : t0;
-                        const float fr = pos - (float)t0;
-                        const int kr = (band * P.nQ + g) * rowlen, ir = band * rowlen;
-                        const int t = TAB == kTabSliced ? max(0, min(t0 - P.tab_lo, nw - 2)) : t0;
-                        const float k0 = TAB == kTabSliced ? s_kq[kr + t] : kq[kr + t];
-                        const float k1 = TAB == kTabSliced ? s_kq[kr + t + 1] : kq[kr + t + 1];
-                        const float i0 = ibt[ir + t], i1 = ibt[ir + t + 1];
-                        kap = fmaf(fr, k1 - k0, k0);
-                        ib = fmaf(fr, i1 - i0, i0);
-                    }
-                    const float iem = P.gen_lds ? s_iem[band * P.nQ + g] : P.inv_emax[band * P.nQ + g];
-                    const float w = P.Ibmax * (kap * ib) * iem;
-                    const float wi = w * (1.0f / ib);
-                    rw = GEN ? w : w * ratio;
-                    rwi = GEN ? wi : wi * ratio;
-                }
-                const float ut = ((float)mt + P.u_off) * kTwoM24;
-                const float up = ((float)mp + P.u_off) * kTwoM24;
-                const float cth = 1.0f - 2.0f * ut;
-                const float sth = 2.0f * sqrtf(ut * (1.0f - ut));
-                float sp, cp;
-                sincos2pi(up, &sp, &cp);
-                const float s1 = sth * cp, s2 = sth * sp;
-                sx = P.polar == 0 ? cth : (P.polar == 1 ? s2 : s1);
-                sy = P.polar == 0 ? s1 : (P.polar == 1 ? cth : s2);
-                sz = P.polar == 0 ? s2 : (P.polar == 1 ? s1 : cth);
-                if (sx == 0.0f) sx = 1e-10f;
-                if (sy == 0.0f) sy = 1e-10f;
-                if (sz == 0.0f) sz = 1e-10f;
-                fl = (sx < 0.0f ? 1 : 0) | (sy < 0.0f ? 2 : 0) | (sz < 0.0f ? 4 : 0);
-                ix = 1.0f / sx;
-                iy = 1.0f / sy;
-                iz = 1.0f / sz;
-                cx = (si.w & 1023) + 1;
-                cy = ((si.w >> 10) & 1023) + 1;
-                cz = ((si.w >> 20) & 1023) + 1;
+                Tc = rd.Tc;
+                band = rd.bg & 0xFFFF;
+                fl = (rd.bg >> 24) & 7;
+                sx = rd.sx; sy = rd.sy; sz = rd.sz;
+                ix = rd.ix; iy = rd.iy; iz = rd.iz;
+                rw = rd.rw;
+                rwi = rd.rwi;
+                const int g = (rd.bg >> 16) & 0xFF;
+                cx = (rd.cell & 1023) + 1;
+                cy = ((rd.cell >> 10) & 1023) + 1;
+                cz = ((rd.cell >> 20) & 1023) + 1;
                 px = s_cent[P.coff[0] + cx];
                 py = s_cent[P.coff[1] + cy];
                 pz = s_cent[P.coff[2] + cz];
@@ -713,7 +720,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 fx = s_face[fo0 + cx - (fl & 1)];
                 fy = s_face[fo1 + cy - ((fl >> 1) & 1)];
                 fz = s_face[fo2 + cz - ((fl >> 2) & 1)];
-                idx = si.z;
+                idx = rd.idx;
                 tau = 0.0f;
                 tra = 1.0f;
                 G = 0.0f;
@@ -723,6 +730,106 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 hash = 2166136261u;
                 rsteps = 0;
                 alive = true;
+            };
+            if (GEN) {
+                // ---- from the wave's queue of drawn rays: queue lane j holds ray
+                // gbase + j of snapshot gsnap (j < gn), and the idle lane of rank r
+                // takes queue lane ghead + r.  An exhausted queue is redrawn from
+                // the next (up to) 64 rays of the claimed super-chunk, one ray per
+                // lane, so the draw runs on whole waves rather than on the idle
+                // lanes of each refill.
+                const uint32_t t1 = min((uint32_t)ni, gn - ghead);
+                {
+                    const bool s1 = idle && rank < t1;
+                    const RayStart rd = pull_ray(Qr, (int)((ghead + (s1 ? rank : 0u)) & 63u));
+                    if (s1) start_ray(rd, gsnap);
+                }
+                ghead += t1;
+                if (t1 < (uint32_t)ni) {
+                    if (cur >= cend && more) {
+                        // claim the next super-chunk, from this XCD's share first,
+                        // then from the following XCDs' shares.  Claim c of XCD X is
+                        // chunk X*per_x + c % per_x of snapshot c / per_x.
+                        bool got = false;
+                        while (xi < nparts) {
+                            const uint32_t X = (xcd + xi) % nparts;
+                            uint32_t c = 0;
+                            if (lane == 0) c = atomicAdd(P.claim + 32u * X, 1u);
+                            c = __builtin_amdgcn_readfirstlane(c);
+                            if (c >= nclaim) {
+                                ++xi;
+                                continue;
+                            }
+                            const uint32_t sn = c / per_x;
+                            const uint32_t lc = X * per_x + (c - sn * per_x);
+                            if (lc >= cps) continue;   // past the snapshot's last chunk
+                            csnap = sn;
+                            cur = lc * S;
+                            cend = cur + S < nrays ? cur + S : nrays;
+                            got = true;
+                            break;
+                        }
+                        if (!got) more = false;
+                    }
+                    if (cur < cend) {
+                        gn = min(64u, cend - cur);
+                        gsnap = csnap;
+                        if ((uint32_t)lane < gn) Qr = draw_ray(cur + (uint32_t)lane, gsnap);
+                        cur += gn;
+                        const bool s2 = idle && rank >= t1 && rank - t1 < gn;
+                        const RayStart rd = pull_ray(Qr, (int)(s2 ? rank - t1 : 0u));
+                        if (s2) start_ray(rd, gsnap);
+                        ghead = min((uint32_t)ni - t1, gn);
+                    }
+                }
+            } else {
+                // ---- hand out consecutive rays to the idle lanes, in lane order
+                const uint32_t avail = cend - cur;
+                uint32_t q = 0xFFFFFFFFu, qs = csnap;
+                if (idle && rank < avail) q = cur + rank;
+                if (avail < (uint32_t)ni && more) {
+                    // the current super-chunk runs out: claim the next one (as above)
+                    bool got = false;
+                    uint32_t nstart = 0, nend = 0, nsn = 0;
+                    while (xi < nparts) {
+                        const uint32_t X = (xcd + xi) % nparts;
+                        uint32_t c = 0;
+                        if (lane == 0) c = atomicAdd(P.claim + 32u * X, 1u);
+                        c = __builtin_amdgcn_readfirstlane(c);
+                        if (c >= nclaim) {
+                            ++xi;
+                            continue;
+                        }
+                        const uint32_t sn = c / per_x;
+                        const uint32_t lc = X * per_x + (c - sn * per_x);
+                        if (lc >= cps) continue;   // past the snapshot's last chunk
+                        nsn = sn;
+                        nstart = lc * S;
+                        nend = nstart + S < nrays ? nstart + S : nrays;
+                        got = true;
+                        break;
+                    }
+                    if (got) {
+                        if (idle && rank >= avail && nstart + (rank - avail) < nend) {
+                            q = nstart + (rank - avail);
+                            qs = nsn;
+                        }
+                        const uint32_t used = (uint32_t)ni - avail;
+                        cur = nstart + used < nend ? nstart + used : nend;
+                        cend = nend;
+                        csnap = nsn;
+                    } else {
+                        more = false;
+                        cur = cend;
+                    }
+                } else {
+                    cur += avail < (uint32_t)ni ? avail : (uint32_t)ni;
+                }
+                // ---- start the handed-out rays
+                if (q != 0xFFFFFFFFu) {
+                    qcur = q;
+                    start_ray(draw_ray(q, qs), qs);
+                }
             }
         }
         if (!alive) continue;
