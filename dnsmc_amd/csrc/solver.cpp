@@ -520,7 +520,9 @@ template <int SPEC, int TAB, int TM, bool REC, bool GEN = false>
 static hipError_t launch_trace_t(bool amc, int ncu, int max_blocks, size_t lds, hipStream_t st, const TraceArgs& P)
 {
     void (*k)(TraceArgs) = amc ? trace<SPEC, TAB, TM, REC, true, GEN> : trace<SPEC, TAB, TM, REC, false, GEN>;
-    const int threads = TAB == kTabSliced ? 1024 : 256;
+    int threads = TAB == kTabSliced ? 1024 : 256;
+    if (TAB == kTabSliced)
+        if (const char* be = getenv("DNSMC_BLOCK")) threads = std::max(64, std::min(1024, atoi(be) / 64 * 64));
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, lds);
     if (e != hipSuccess) return e;
@@ -782,10 +784,12 @@ static int do_run(dnsmc_solver* s)
     P.slots_per_emitter = spe;
     P.fd_R = make_fastdiv((uint32_t)R);
     {
-        // measured defaults: the fused launch refills at 24 idle lanes and
-        // hands out 512-ray super-chunks; the batched launches 16 and 256
+        // measured defaults: the fused and snapshot-wise launches refill at 24
+        // idle lanes (40 with coarsening levels, whose rays are short, so the
+        // refill weighs more) and hand out 512-ray super-chunks; the batched
+        // launches 16 and 256
         const char* rk = getenv("DNSMC_REFILL_K");
-        P.refill_k = rk ? atoi(rk) : (fused || snapwise ? 24 : 16);
+        P.refill_k = rk ? atoi(rk) : (fused || snapwise ? (P.nlev > 1 ? 40 : 24) : 16);
         if (P.refill_k < 1) P.refill_k = 1;
         if (P.refill_k > 64) P.refill_k = 64;
         const char* sc = getenv("DNSMC_SUPER");

@@ -178,7 +178,9 @@ int dnsmc_set_emitters_list(dnsmc_solver *s, const int *cells, int ncells);
  *   DNSMC_PIPES=1..3  pipeline streams of the batched modes
  *   DNSMC_TABLES=global  keep kappa / Planck tables out of LDS
  *   DNSMC_REFILL_K=k  idle lanes of a wave before it starts new rays
- *   DNSMC_SUPER=n     rays per claimed super-chunk (multiple of 64) */
+ *   DNSMC_SUPER=n     rays per claimed super-chunk (multiple of 64)
+ *   DNSMC_BLOCK=n     threads per block of the traversal with LDS table
+ *                     slices (default 1024: one block and 16 waves per CU) */
 int dnsmc_run(dnsmc_solver *s);
 /* Wait for the queued solve. */
 int dnsmc_wait(dnsmc_solver *s);
