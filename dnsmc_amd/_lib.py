@@ -25,6 +25,7 @@ EXPORTED = [
     "dnsmc_get_rays", "dnsmc_get_ray_records", "dnsmc_set_record", "dnsmc_sync", "dnsmc_bench_copy",
     "dnsmc_mlp_nparams", "dnsmc_mlp_samples", "dnsmc_mlp_features", "dnsmc_mlp_grad", "dnsmc_mlp_adam",
     "dnsmc_mlp_forward", "dnsmc_dev_alloc", "dnsmc_dev_free", "dnsmc_dev_copy",
+    "dnsmc_fix_temp_slab", "dnsmc_commit_field", "dnsmc_fix_rad_slab",
 ]
 
 
@@ -97,6 +98,10 @@ def lib():
     L.dnsmc_set_tables.argtypes = [vp, i32, i32, i32] + [f32p] * 9
     L.dnsmc_set_field.argtypes = [vp, i32, vp, i32]
     L.dnsmc_set_mlp.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_float]
+    i32p = C.POINTER(C.c_int)
+    L.dnsmc_fix_temp_slab.argtypes = [vp, i32, f64p, i32p, i32p, C.c_double, C.c_double]
+    L.dnsmc_commit_field.argtypes = [vp, i32]
+    L.dnsmc_fix_rad_slab.argtypes = [vp, i32, i32p, i32p, C.c_double, f64p, f64p]
     L.dnsmc_set_emitters_slab.argtypes = [vp, i32]
     L.dnsmc_set_emitters_list.argtypes = [vp, C.POINTER(C.c_int), i32]
     L.dnsmc_run.argtypes = [vp]

@@ -1330,6 +1330,45 @@ __global__ void copy_f32x4(const float4* __restrict__ src, float4* __restrict__ 
         dst[i] = src[i];
 }
 
+// ------------------------------------------------- DNS coupling slabs ----
+// fix_temp (code/fix_var.f:49-73) for one rank's slab, written straight into
+// the bricked device field instead of summed into a full-size host buffer and
+// all-reduced: T(i,j,k) = cnew * Tc / Tplus + Tc in FP64 (left to right, as
+// the Fortran expression), rounded to FP32 as the real(4) TMC is.  cnew is the
+// rank's array cnew(0:n0+1, nj, nk), i fastest; global (i, j, k) of the slab
+// (1-based, i0..i0+ni-1 etc.) reads cnew(i - i0 + 1, j - j0 + 1, k - k0 + 1).
+// Ghost indices (0 and n+1) are not part of the bricked field and are skipped.
+__global__ void slab_to_field(const double* cnew, int n0, int n1, int n2, int i0, int j0, int k0, int ni, int nj,
+                              int nk, double Tc, double Tplus, float* T, int nbx, int nby)
+{
+    const long long n = (long long)ni * nj * nk;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        const int a = (int)(c % ni), b = (int)((c / ni) % nj), d = (int)(c / ((long long)ni * nj));
+        const int i = i0 + a, j = j0 + b, k = k0 + d;
+        if (i < 1 || i > n0 || j < 1 || j > n1 || k < 1 || k > n2) continue;
+        const double v = cnew[(long long)(a + 1) + (long long)(n0 + 2) * (b + (long long)nj * d)];
+        T[brick_index(i - 1, j - 1, k - 1, nbx, nby)] = (float)(v * Tc / Tplus + Tc);
+    }
+}
+
+// fix_rad (code/fix_var.f:75-121) for one rank's k range, from the device
+// results: divQ(i, jl, k - k0 + 1) = Q / stefan / Tc^4 and vcore likewise from
+// the standard error, for the emitters in the solver's j slab [jlo, jlo + nj)
+// (0-based) and k in [k0, k0 + nk) (0-based); out arrays (0:n0+1, nj, nk).
+__global__ void field_to_slab(const int* emit, long long nemit, const double* Q, const double* sd, int n0, int n1,
+                              int jlo, int nj, int k0, int nk, double stefan, double tc4, double* divq, double* vcore)
+{
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= nemit) return;
+    const int cell = emit[e];
+    const int i = cell % n0, j = (cell / n0) % n1, k = cell / (n0 * n1);
+    const int jl = j - jlo, kl = k - k0;
+    if (jl < 0 || jl >= nj || kl < 0 || kl >= nk) return;
+    const long long o = (long long)(i + 1) + (long long)(n0 + 2) * (jl + (long long)nj * kl);
+    divq[o] = Q[e] / stefan / tc4;
+    vcore[o] = sd[e] / stefan / tc4;
+}
+
 __global__ void fill_u32(unsigned* p, long long n, unsigned v)
 {
     long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;

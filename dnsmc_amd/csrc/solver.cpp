@@ -55,6 +55,10 @@ __global__ void mlp_samples(FeatArgs f, const int4* slotinfo, long long slot0, i
                             int* tb, float* wt);
 __global__ void mlp_features(FeatArgs f, const int4* slotinfo, long long n, float* X);
 __global__ void fill_u32(unsigned* p, long long n, unsigned v);
+__global__ void slab_to_field(const double* cnew, int n0, int n1, int n2, int i0, int j0, int k0, int ni, int nj,
+                              int nk, double Tc, double Tplus, float* T, int nbx, int nby);
+__global__ void field_to_slab(const int* emit, long long nemit, const double* Q, const double* sd, int n0, int n1,
+                              int jlo, int nj, int k0, int nk, double stefan, double tc4, double* divq, double* vcore);
 __global__ void copy_f32x4(const float4* src, float4* dst, long long n4);
 }  // namespace dn
 
@@ -123,7 +127,9 @@ struct dnsmc_solver {
     std::vector<float> Tmax_snap;
     std::vector<float> Tlo_snap, Thi_snap;   // range over all levels of each snapshot
     std::vector<char> field_set;
+    std::vector<char> slab_open;   // slabs written since the last commit
     DevBuf<unsigned> d_lohi;
+    DevBuf<double> d_slab;         // staging of the DNS coupling slabs
     DevBuf<float> d_T[kMaxLev];
     long long ncell[kMaxLev] = {0, 0, 0, 0, 0, 0};     // cells per level
     long long tstride[kMaxLev] = {0, 0, 0, 0, 0, 0};   // bricked floats per level and snapshot
@@ -1098,6 +1104,53 @@ static int fetch_perm(const T* dev, long long c, const std::vector<long long>& p
     return DNSMC_OK;
 }
 
+// After the fine field of snapshot `snap` is on the device: coarse levels by
+// the spline passes, the temperature range of every level (LDS table slice),
+// Tmax and the positivity check (code/mc_gpu.cu:456-480).
+static int commit_field(dnsmc_solver* s, int snap)
+{
+    int rc = run_spline(s, snap);
+    if (rc) return rc;
+    // temperature range over every level (the spline can leave the fine range)
+    HIPCHK(s->d_lohi.ensure(2 * s->lev.size()));
+    for (size_t l = 0; l < s->lev.size(); ++l) {
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s->stream, s->d_lohi.p + 2 * l, 1ll, 0xFFFFFFFFu);
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s->stream, s->d_lohi.p + 2 * l + 1, 1ll, 0u);
+        const int* ln = s->lev[l].n;
+        const long long ncl = (long long)ln[0] * ln[1] * ln[2];
+        const int nb = (int)std::min<long long>((ncl + 255) / 256, (long long)s->ncu * 16);
+        hipLaunchKernelGGL(t_range, dim3(nb), dim3(256), 0, s->stream, s->d_T[l].p + (size_t)snap * s->tstride[l],
+                           ln[0], ln[1], ln[2], s->nbx[l], s->nby[l], s->d_lohi.p + 2 * l);
+        HIPCHK(hipGetLastError());
+    }
+    std::vector<unsigned> lohi(2 * s->lev.size());
+    HIPCHK(hipMemcpyAsync(lohi.data(), s->d_lohi.p, lohi.size() * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    auto key_float = [](unsigned k) {
+        unsigned u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+        float f;
+        memcpy(&f, &u, sizeof f);
+        return f;
+    };
+    float lo = 0.f, hi = 0.f;
+    for (size_t l = 0; l < s->lev.size(); ++l) {
+        const float a = key_float(lohi[2 * l]), b = key_float(lohi[2 * l + 1]);
+        lo = l == 0 ? a : std::min(lo, a);
+        hi = l == 0 ? b : std::max(hi, b);
+    }
+    s->Tlo_snap[snap] = lo;
+    s->Thi_snap[snap] = hi;
+    s->Tmax_snap[snap] = key_float(lohi[1]);   // level-0 maximum
+    s->slab_open[snap] = 0;
+    // fine field: positive and finite everywhere (a NaN shows in the level-0 bounds)
+    if (!(key_float(lohi[0]) > 0.f) || !std::isfinite(key_float(lohi[1]))) {
+        s->field_set[snap] = 0;
+        return fail(DNSMC_ERANGE, "temperature field holds a non-positive or non-finite value");
+    }
+    s->field_set[snap] = 1;
+    return DNSMC_OK;
+}
+
 // ================================================================ C ABI ====
 extern "C" {
 
@@ -1147,6 +1200,7 @@ int dnsmc_create(const dnsmc_config* cfg, dnsmc_solver** out)
     s->Tlo_snap.assign(cfg->nsnap, 0.f);
     s->Thi_snap.assign(cfg->nsnap, 0.f);
     s->field_set.assign(cfg->nsnap, 0);
+    s->slab_open.assign(cfg->nsnap, 0);
     rc = upload_geometry(s);
     if (rc) { dnsmc_destroy(s); return rc; }
     // default emitters: every cell
@@ -1261,39 +1315,93 @@ int dnsmc_set_field(dnsmc_solver* s, int snap, const float* T, int is_device)
         }
     HIPCHK(hipMemcpyAsync(s->d_T[0].p + (size_t)snap * s->tstride[0], in.data(), in.size() * sizeof(float),
                           hipMemcpyHostToDevice, s->stream));
-    int rc = run_spline(s, snap);
+    int rc = commit_field(s, snap);
     if (rc) return rc;
-    // temperature range over every level (the spline can leave the fine range)
-    HIPCHK(s->d_lohi.ensure(2 * s->lev.size()));
-    for (size_t l = 0; l < s->lev.size(); ++l) {
-        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s->stream, s->d_lohi.p + 2 * l, 1ll, 0xFFFFFFFFu);
-        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s->stream, s->d_lohi.p + 2 * l + 1, 1ll, 0u);
-        const int* ln = s->lev[l].n;
-        const long long ncl = (long long)ln[0] * ln[1] * ln[2];
-        const int nb = (int)std::min<long long>((ncl + 255) / 256, (long long)s->ncu * 16);
-        hipLaunchKernelGGL(t_range, dim3(nb), dim3(256), 0, s->stream, s->d_T[l].p + (size_t)snap * s->tstride[l],
-                           ln[0], ln[1], ln[2], s->nbx[l], s->nby[l], s->d_lohi.p + 2 * l);
-        HIPCHK(hipGetLastError());
-    }
-    std::vector<unsigned> lohi(2 * s->lev.size());
-    HIPCHK(hipMemcpyAsync(lohi.data(), s->d_lohi.p, lohi.size() * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    auto key_float = [](unsigned k) {
-        unsigned u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
-        float f;
-        memcpy(&f, &u, sizeof f);
-        return f;
-    };
-    float lo = 0.f, hi = 0.f;
-    for (size_t l = 0; l < s->lev.size(); ++l) {
-        const float a = key_float(lohi[2 * l]), b = key_float(lohi[2 * l + 1]);
-        lo = l == 0 ? a : std::min(lo, a);
-        hi = l == 0 ? b : std::max(hi, b);
-    }
-    s->Tlo_snap[snap] = lo;
-    s->Thi_snap[snap] = hi;
     s->Tmax_snap[snap] = tmax;
-    s->field_set[snap] = 1;
+    return DNSMC_OK;
+}
+
+int dnsmc_fix_temp_slab(dnsmc_solver* s, int snap, const double* cnew, const int* startc, const int* endc, double Tc,
+                        double Tplus)
+{
+    if (!s || !cnew || !startc || !endc) return fail(DNSMC_EINVAL, "null argument");
+    if (snap < 0 || snap >= s->cfg.nsnap) return fail(DNSMC_EINVAL, "snapshot index out of range");
+    if (!(Tplus != 0.0) || !std::isfinite(Tplus) || !std::isfinite(Tc)) return fail(DNSMC_EINVAL, "Tc and Tplus must be finite, Tplus nonzero");
+    const int* n = s->cfg.n;
+    int len[3];
+    for (int a = 0; a < 3; ++a) {
+        if (startc[a] < 1 || endc[a] > n[a] || endc[a] < startc[a])
+            return fail(DNSMC_EINVAL, "slab index range outside 1..n or empty");
+        len[a] = endc[a] - startc[a] + 1;
+    }
+    if (s->running) {
+        int rc = do_wait(s);
+        if (rc) return rc;
+    }
+    float* T = s->d_T[0].p + (size_t)snap * s->tstride[0];
+    if (!s->slab_open[snap]) {
+        // first slab since the last commit: every cell starts uncovered (NaN),
+        // so a field the slabs do not cover fails the commit's check
+        const long long nf = s->tstride[0];
+        hipLaunchKernelGGL(fill_u32, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s->stream, (unsigned*)T, nf,
+                           0x7FC00000u);
+        HIPCHK(hipGetLastError());
+        s->slab_open[snap] = 1;
+        s->field_set[snap] = 0;
+    }
+    // cnew(0:n0+1, nj, nk) of the rank, i fastest
+    const size_t cnt = (size_t)(n[0] + 2) * len[1] * len[2];
+    HIPCHK(s->d_slab.ensure(cnt));
+    HIPCHK(hipMemcpyAsync(s->d_slab.p, cnew, cnt * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    const long long nc = (long long)len[0] * len[1] * len[2];
+    const int nb = (int)std::min<long long>((nc + 255) / 256, (long long)s->ncu * 16);
+    hipLaunchKernelGGL(slab_to_field, dim3(nb), dim3(256), 0, s->stream, (const double*)s->d_slab.p, n[0], n[1], n[2],
+                       startc[0], startc[1], startc[2], len[0], len[1], len[2], Tc, Tplus, T, s->nbx[0], s->nby[0]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));   // cnew and the staging buffer are free again
+    return DNSMC_OK;
+}
+
+int dnsmc_commit_field(dnsmc_solver* s, int snap)
+{
+    if (!s) return fail(DNSMC_EINVAL, "null solver");
+    if (snap < 0 || snap >= s->cfg.nsnap) return fail(DNSMC_EINVAL, "snapshot index out of range");
+    if (!s->slab_open[snap]) return fail(DNSMC_ESTATE, "no slab written since the last commit");
+    if (s->running) {
+        int rc = do_wait(s);
+        if (rc) return rc;
+    }
+    return commit_field(s, snap);
+}
+
+int dnsmc_fix_rad_slab(dnsmc_solver* s, int snap, const int* startc, const int* endc, double Tc, double* divQ,
+                       double* vcore)
+{
+    if (!s || !startc || !endc || !divQ || !vcore) return fail(DNSMC_EINVAL, "null argument");
+    if (!(Tc > 0.0) || !std::isfinite(Tc)) return fail(DNSMC_EINVAL, "Tc must be positive and finite");
+    const int* n = s->cfg.n;
+    if (startc[2] < 1 || endc[2] > n[2] || endc[2] < startc[2]) return fail(DNSMC_EINVAL, "k range outside 1..n2 or empty");
+    int rc = do_wait(s);
+    if (rc) return rc;
+    if (snap < 0 || snap >= s->cfg.nsnap || s->last_nemit == 0) return fail(DNSMC_ESTATE, "no results for snapshot");
+    const int nj = n[1] / s->cfg.p_row, nk = endc[2] - startc[2] + 1;
+    const size_t cnt = (size_t)(n[0] + 2) * nj * nk;   // divQ(0:n0+1, nj, nk)
+    HIPCHK(s->d_slab.ensure(2 * cnt));
+    double* dq = s->d_slab.p;
+    double* dv = s->d_slab.p + cnt;
+    hipLaunchKernelGGL(fill_f64, dim3((unsigned)((2 * cnt + 255) / 256)), dim3(256), 0, s->stream, dq, (long long)(2 * cnt),
+                       0.0);
+    const long long ne = s->last_nemit;
+    const double stefan = 5.670373e-08;   // code/param.txt:19
+    const double tc4 = Tc * Tc * Tc * Tc;
+    hipLaunchKernelGGL(field_to_slab, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s->stream,
+                       (const int*)s->d_emit.p, ne, (const double*)(s->d_Q.p + (size_t)snap * ne),
+                       (const double*)(s->d_sd.p + (size_t)snap * ne), n[0], n[1], s->ystart - 1, nj, startc[2] - 1, nk,
+                       stefan, tc4, dq, dv);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(divQ, dq, cnt * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(vcore, dv, cnt * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     return DNSMC_OK;
 }
 

@@ -108,6 +108,38 @@ class Solver:
             raise ValueError("T must have shape (n0+2, n1+2, n2+2), got %s" % (T.shape,))
         check(lib().dnsmc_set_field(self.h, snap, T.ctypes.data, 0))
 
+    def fix_temp_slab(self, snap, cnew, startc, endc, Tc, Tplus):
+        """One rank's slab of the nondimensional temperature into the device
+        field (fix_temp, code/fix_var.f:49-73, without the all-reduce): cnew is
+        the rank's array cnew(0:n0+1, nj, nk) (Fortran order, float64), startc
+        / endc the 1-based inclusive global ranges.  Call commit_field after
+        the last slab."""
+        st = np.ascontiguousarray(startc, np.int32)
+        en = np.ascontiguousarray(endc, np.int32)
+        c = np.asfortranarray(cnew, dtype=np.float64)
+        shp = (self.n[0] + 2, int(en[1] - st[1] + 1), int(en[2] - st[2] + 1))
+        if c.shape != shp:
+            raise ValueError("cnew must have shape %s, got %s" % (shp, c.shape))
+        i32p = C.POINTER(C.c_int)
+        check(lib().dnsmc_fix_temp_slab(self.h, snap, _f64(c), st.ctypes.data_as(i32p), en.ctypes.data_as(i32p),
+                                        float(Tc), float(Tplus)))
+
+    def commit_field(self, snap):
+        check(lib().dnsmc_commit_field(self.h, snap))
+
+    def fix_rad_slab(self, snap, startc, endc, Tc):
+        """divQ and vcore of one rank (fix_rad, code/fix_var.f:75-121): arrays
+        (0:n0+1, n1/p_row, nk) in Fortran order, FP64."""
+        st = np.ascontiguousarray(startc, np.int32)
+        en = np.ascontiguousarray(endc, np.int32)
+        shp = (self.n[0] + 2, self.n[1] // self.cfg.p_row, int(en[2] - st[2] + 1))
+        dq = np.zeros(shp, np.float64, order="F")
+        vc = np.zeros(shp, np.float64, order="F")
+        i32p = C.POINTER(C.c_int)
+        check(lib().dnsmc_fix_rad_slab(self.h, snap, st.ctypes.data_as(i32p), en.ctypes.data_as(i32p), float(Tc),
+                                       _f64(dq), _f64(vc)))
+        return dq, vc
+
     def set_emitters_slab(self, ystart):
         check(lib().dnsmc_set_emitters_slab(self.h, ystart))
         self.nemit = self.n[0] * (self.n[1] // self.cfg.p_row) * self.n[2]

@@ -149,6 +149,33 @@ int dnsmc_set_tables(dnsmc_solver *s, int nB, int nQ, int nT, const float *Tnb, 
 /* Temperature field of snapshot `snap`, Fortran layout with ghosts
  * (n0+2)(n1+2)(n2+2), host or device pointer (is_device). */
 int dnsmc_set_field(dnsmc_solver *s, int snap, const float *T, int is_device);
+/* DNS coupling without the host all-reduces (code/fix_var.f:49-121, called
+ * from code/main_comp.f around mc_gpu_ / get_results_).
+ * dnsmc_fix_temp_slab replaces fix_temp (code/fix_var.f:49-73): one rank's
+ * slab of the nondimensional temperature goes straight into snapshot `snap`
+ * of the device field, T = cnew * Tc / Tplus + Tc evaluated in FP64 left to
+ * right and rounded to FP32 (the value the reference's MPI_SUM all-reduce of
+ * disjoint slabs leaves in the real(4) TMC).  cnew is the rank's x-pencil
+ * array cnew(0:n0+1, nj, nk), i fastest, host memory, with nj, nk the extents
+ * of [startc, endc] (1-based inclusive global indices, the 2DECOMP
+ * xstart/xend; interior only).  The first slab after a commit marks every
+ * cell uncovered; dnsmc_commit_field, after the last slab, builds the coarse
+ * levels and the temperature range and fails with DNSMC_ERANGE if any cell
+ * was left uncovered or is non-positive.  A node's GPU owner calls it for its
+ * own slab and for the slabs the other ranks send it (a gather of slabs
+ * instead of the all-reduce of the full field).
+ * dnsmc_fix_rad_slab replaces fix_rad (code/fix_var.f:75-121) for one rank:
+ * divQ(i, jl, k - startc[2] + 1) = Q / stefan / Tc^4 and vcore likewise from
+ * the standard error, stefan = 5.670373e-08 (code/param.txt:19), Tc^4 as
+ * Tc*Tc*Tc*Tc, all FP64 and from the FP64 results (the reference divides the
+ * real(4) copy); i = 1..n0, jl = 1..n1/p_row over the emitter slab
+ * (dnsmc_set_emitters_slab), k = startc[2]..endc[2]; host arrays
+ * (0:n0+1, n1/p_row, endc[2]-startc[2]+1), zero outside those cells. */
+int dnsmc_fix_temp_slab(dnsmc_solver *s, int snap, const double *cnew, const int *startc, const int *endc,
+                        double Tc, double Tplus);
+int dnsmc_commit_field(dnsmc_solver *s, int snap);
+int dnsmc_fix_rad_slab(dnsmc_solver *s, int snap, const int *startc, const int *endc, double Tc, double *divQ,
+                       double *vcore);
 /* Learned proposal (no counterpart in the reference): network 6 -> 64 -> 64 ->
  * (64 direction bins | nbin = nB*nQ (band, g) bins, padded to noutp, a multiple
  * of 64).  Weights as BF16 bits, row-major [in][out] (W1 6x64, W2 64x64,
