@@ -26,6 +26,7 @@ EXPORTED = [
     "dnsmc_mlp_nparams", "dnsmc_mlp_samples", "dnsmc_mlp_features", "dnsmc_mlp_grad", "dnsmc_mlp_adam",
     "dnsmc_mlp_forward", "dnsmc_dev_alloc", "dnsmc_dev_free", "dnsmc_dev_copy",
     "dnsmc_fix_temp_slab", "dnsmc_commit_field", "dnsmc_fix_rad_slab",
+    "fix_temp_gpu_", "mc_gpu_commit_", "fix_rad_gpu_",
 ]
 
 
@@ -131,6 +132,12 @@ def lib():
     L.mc_gpu_.argtypes = [f32p, C.POINTER(C.c_int)]
     L.mc_gpu_.restype = None
     L.get_results_.argtypes = [f32p, f32p]
+    L.fix_temp_gpu_.argtypes = [f64p, C.POINTER(C.c_int), C.POINTER(C.c_int), f64p, f64p]
+    L.fix_temp_gpu_.restype = None
+    L.mc_gpu_commit_.argtypes = [C.POINTER(C.c_int)]
+    L.mc_gpu_commit_.restype = None
+    L.fix_rad_gpu_.argtypes = [f64p, f64p, C.POINTER(C.c_int), C.POINTER(C.c_int), f64p]
+    L.fix_rad_gpu_.restype = None
     L.get_results_.restype = None
     _lib = L
     return L

@@ -1853,6 +1853,44 @@ void get_results_(float* resfort, float* varfort)
     if (dnsmc_get_results_fortran(g_legacy, 0, resfort, varfort)) legacy_die("get_results_");
 }
 
+// The DNS coupling without the host all-reduces, in the legacy conventions
+// (Fortran-callable, arguments by reference, exit(1) with a message on error),
+// on the solver mc_gpu_ uses: fix_temp_gpu_ per rank's slab, then
+// mc_gpu_commit_ in place of mc_gpu_; fix_rad_gpu_ per rank after the solve.
+void fix_temp_gpu_(double* cnew, int* startc, int* endc, double* Tc, double* Tplus)
+{
+    std::lock_guard<std::mutex> lk(g_legacy_mu);
+    dnsmc_solver* s = legacy_solver();
+    if (!cnew || !startc || !endc || !Tc || !Tplus) {
+        g_err = "null argument";
+        legacy_die("fix_temp_gpu_");
+    }
+    if (dnsmc_fix_temp_slab(s, 0, cnew, startc, endc, *Tc, *Tplus)) legacy_die("fix_temp_gpu_");
+}
+
+void mc_gpu_commit_(int* ystart)
+{
+    std::lock_guard<std::mutex> lk(g_legacy_mu);
+    dnsmc_solver* s = legacy_solver();
+    if (dnsmc_commit_field(s, 0)) legacy_die("mc_gpu_commit_: field");
+    if (dnsmc_set_emitters_slab(s, ystart ? *ystart : 1)) legacy_die("mc_gpu_commit_: ystart");
+    if (dnsmc_run(s)) legacy_die("mc_gpu_commit_: launch");
+}
+
+void fix_rad_gpu_(double* divQ, double* vcore, int* startc, int* endc, double* Tc)
+{
+    std::lock_guard<std::mutex> lk(g_legacy_mu);
+    if (!g_legacy) {
+        g_err = "fix_rad_gpu_ called before a solve";
+        legacy_die("fix_rad_gpu_");
+    }
+    if (!Tc) {
+        g_err = "null argument";
+        legacy_die("fix_rad_gpu_");
+    }
+    if (dnsmc_fix_rad_slab(g_legacy, 0, startc, endc, *Tc, divQ, vcore)) legacy_die("fix_rad_gpu_");
+}
+
 void mc_gpu(float* Tfort, int* ystart) { mc_gpu_(Tfort, ystart); }
 void get_results(float* resfort, float* varfort) { get_results_(resfort, varfort); }
 

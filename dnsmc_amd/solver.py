@@ -232,9 +232,34 @@ def get_results(Q, V):
     lib().get_results_(_f32(Q), _f32(V))
 
 
+def fix_temp_gpu(cnew, startc, endc, Tc, Tplus):
+    """fix_temp_gpu_ (replaces code/fix_var.f:49-73): one rank's slab into the
+    legacy solver's field; cnew(0:n0+1, nj, nk) float64, Fortran order."""
+    c = np.asfortranarray(cnew, dtype=np.float64)
+    st = (C.c_int * 3)(*[int(x) for x in startc])
+    en = (C.c_int * 3)(*[int(x) for x in endc])
+    lib().fix_temp_gpu_(_f64(c), st, en, C.byref(C.c_double(Tc)), C.byref(C.c_double(Tplus)))
+
+
+def mc_gpu_commit(ystart=1):
+    """mc_gpu_commit_: queue the solve of the field the slabs built."""
+    lib().mc_gpu_commit_(C.byref(C.c_int(ystart)))
+
+
+def fix_rad_gpu(startc, endc, Tc, shape):
+    """fix_rad_gpu_ (replaces code/fix_var.f:75-121): one rank's divQ, vcore
+    of the last solve, arrays of `shape` = (n0+2, n1/p_row, nk), float64."""
+    dq = np.zeros(shape, np.float64, order="F")
+    vc = np.zeros(shape, np.float64, order="F")
+    st = (C.c_int * 3)(*[int(x) for x in startc])
+    en = (C.c_int * 3)(*[int(x) for x in endc])
+    lib().fix_rad_gpu_(_f64(dq), _f64(vc), st, en, C.byref(C.c_double(Tc)))
+    return dq, vc
+
+
 def sync():
     check(lib().dnsmc_sync())
 
 
 __all__ = ["Solver", "default_config", "parse_config", "config_from_dict", "mc_gpu", "get_results", "sync",
-           "_lib"]
+           "fix_temp_gpu", "mc_gpu_commit", "fix_rad_gpu", "_lib"]

@@ -48,6 +48,22 @@ void get_results_(float *resfort, float *varfort);
 void mc_gpu(float *Tfort, int *ystart);
 void get_results(float *resfort, float *varfort);
 
+/* Replace the host all-reduces of the DNS coupling, code/fix_var.f:49-73
+ * (subroutine fix_temp) and :75-121 (subroutine fix_rad), called from
+ * code/main_comp.f:154,196,199, on the solver mc_gpu_ uses (same legacy
+ * conventions: by reference, abort with a message on error).
+ * fix_temp_gpu_(cnew, startc, endc, Tc, Tplus): one rank's slab, real(8)
+ *   cnew(0:imax+1, jmax/p_row, kmax/p_col) with the rank's xstart / xend,
+ *   into the solver's temperature field (see dnsmc_fix_temp_slab);
+ * mc_gpu_commit_(ystart): after every rank's slab has been written, queue the
+ *   solve as mc_gpu_(TMC, ystart) would;
+ * fix_rad_gpu_(divQ, vcore, startc, endc, Tc): one rank's real(8)
+ *   divQ / vcore(0:imax+1, jmax/p_row, kmax/p_col) from the last solve (see
+ *   dnsmc_fix_rad_slab); waits for the solve like get_results_. */
+void fix_temp_gpu_(double *cnew, int *startc, int *endc, double *Tc, double *Tplus);
+void mc_gpu_commit_(int *ystart);
+void fix_rad_gpu_(double *divQ, double *vcore, int *startc, int *endc, double *Tc);
+
 /* ------------------------------------------------------------------------
  * Extended API.  No counterpart in the reference (its configuration is
  * compile-time macros in code/param.h and code/NarrowBand.h); the two calls

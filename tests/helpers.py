@@ -134,3 +134,11 @@ def z_scores(Q, Qref, sd, sd_ref=None):
 
 def planck_mean_closed(T):
     return 5.670373e-08 * T ** 4 / math.pi
+
+
+def legacy_conf_text():
+    """DNSMC_CONFIG of the tests that drive the legacy entry points.  Those use
+    one process-wide solver, created from this text at the first call, so
+    every such test shares it."""
+    return ("n = 16, 16, 16\nphotmax = 64\ngrid_num = 2\nmaxs = 5, 100000\ntally = 1\n"
+            "table_dir = %s\n" % particle_table_dir())

@@ -17,7 +17,7 @@ def _declared_functions():
     text = open(os.path.join(ROOT, "include", "dnsmc_rad.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     text = re.sub(r"//[^\n]*", "", text)
-    names = set(re.findall(r"\b((?:dnsmc_\w+)|mc_gpu_?|get_results_?)\s*\(", text))
+    names = set(re.findall(r"\b((?:dnsmc_\w+)|mc_gpu_?|get_results_?|mc_gpu_commit_|fix_temp_gpu_|fix_rad_gpu_)\s*\(", text))
     return sorted(names)
 
 

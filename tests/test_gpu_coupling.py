@@ -96,3 +96,41 @@ def test_fix_rad_slabs(ck_table_dir):
         np.testing.assert_array_equal(vc, coupling.fix_rad(v64, st, en, TC, N[0], nj))
         np.testing.assert_allclose(dq, coupling.fix_rad(q32, st, en, TC, N[0], nj), rtol=1e-7, atol=0)
     S.close()
+
+
+def test_legacy_coupling_calls(tmp_path):
+    """fix_temp_gpu_ per rank + mc_gpu_commit_ + fix_rad_gpu_ on the legacy
+    solver against the reference's sequence fix_temp -> mc_gpu_ ->
+    get_results_ -> fix_rad (code/main_comp.f:192-202) on the same solver."""
+    import os
+    from helpers import legacy_conf_text
+    conf = tmp_path / "rad.conf"
+    conf.write_text(legacy_conf_text())
+    os.environ["DNSMC_CONFIG"] = str(conf)
+    cfg = dnsmc_amd.parse_config(conf.read_text())
+    n = tuple(cfg.n)
+    nj = n[1] // cfg.p_row
+    rng = np.random.default_rng(9)
+    glob = rng.uniform(0.0, 0.5, size=(n[0] + 2, n[1], n[2]))
+    Tc, Tplus = 600.0, 1.0
+    slabs = []
+    for jr in np.array_split(np.arange(1, n[1] + 1), 2):
+        for kr in np.array_split(np.arange(1, n[2] + 1), 2):
+            s = (1, int(jr[0]), int(kr[0]))
+            e = (n[0], int(jr[-1]), int(kr[-1]))
+            slabs.append((np.asfortranarray(glob[:, s[1] - 1:e[1], s[2] - 1:e[2]]), s, e))
+    # reference sequence on the legacy solver
+    tmc = coupling.fix_temp(slabs, n, Tc, Tplus)
+    dnsmc_amd.mc_gpu(tmc, 1)
+    Q = np.zeros((n[0] + 2, nj + 2, n[2] + 2), np.float32, order="F")
+    V = np.zeros_like(Q)
+    dnsmc_amd.get_results(Q, V)
+    # the same step through the slab calls
+    for c, s, e in slabs:
+        dnsmc_amd.fix_temp_gpu(c, s, e, Tc, Tplus)
+    dnsmc_amd.mc_gpu_commit(1)
+    for c, s, e in slabs:
+        dq, vc = dnsmc_amd.fix_rad_gpu(s, e, Tc, (n[0] + 2, nj, e[2] - s[2] + 1))
+        np.testing.assert_allclose(dq, coupling.fix_rad(Q, s, e, Tc, n[0], nj), rtol=1e-7, atol=0)
+        np.testing.assert_allclose(vc, coupling.fix_rad(V, s, e, Tc, n[0], nj), rtol=1e-7, atol=0)
+        assert np.count_nonzero(dq) == n[0] * nj * (e[2] - s[2] + 1)

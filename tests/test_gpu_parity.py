@@ -14,7 +14,7 @@ import pytest
 
 import dnsmc_amd
 from dnsmc_amd import workloads
-from helpers import oracle_problem_dict, particle_table_dir, read_reference_tables, ref_subsequence
+from helpers import legacy_conf_text, oracle_problem_dict, particle_table_dir, read_reference_tables, ref_subsequence
 
 pytestmark = pytest.mark.gpu
 
@@ -203,8 +203,7 @@ def test_grey_slab_closed_form(oracle):
 def test_legacy_abi_fortran_layout(tmp_path):
     """mc_gpu_/get_results_ through the Fortran-layout boundary."""
     conf = tmp_path / "rad.conf"
-    conf.write_text("n = 16, 16, 16\nphotmax = 64\ngrid_num = 2\nmaxs = 5, 100000\ntally = 1\n"
-                    "table_dir = %s\n" % particle_table_dir())
+    conf.write_text(legacy_conf_text())
     os.environ["DNSMC_CONFIG"] = str(conf)
     cfg = dnsmc_amd.parse_config(conf.read_text())
     S = dnsmc_amd.Solver(cfg)
