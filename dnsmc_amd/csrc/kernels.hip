@@ -570,7 +570,10 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     gfloat* Tl = lev_T(s_lev);
     gfloat* T0 = Tl;
     uint32_t rsnap = 0;
-    // GEN: the wave's queue of drawn rays (lane j: ray gbase + j, j < gn)
+    // the wave's queue of drawn rays (lane j: ray gbase + j, j < gn); the
+    // per-ray outputs (TMODE 1) and records index rays by lane, so they keep
+    // the direct hand-out
+    constexpr bool QUEUE = TMODE == 0 && !REC;
     uint32_t gn = 0, ghead = 0, gsnap = 0;
     RayStart Qr = {};
 
@@ -731,13 +734,13 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 rsteps = 0;
                 alive = true;
             };
-            if (GEN) {
+            if (QUEUE) {
                 // ---- from the wave's queue of drawn rays: queue lane j holds ray
                 // gbase + j of snapshot gsnap (j < gn), and the idle lane of rank r
-                // takes queue lane ghead + r.  An exhausted queue is redrawn from
-                // the next (up to) 64 rays of the claimed super-chunk, one ray per
-                // lane, so the draw runs on whole waves rather than on the idle
-                // lanes of each refill.
+                // takes queue lane ghead + r.  An exhausted queue is redrawn (or
+                // re-read from the descriptors) for the next (up to) 64 rays of the
+                // claimed super-chunk, one ray per lane, so that work runs on whole
+                // waves rather than on the idle lanes of each refill.
                 const uint32_t t1 = min((uint32_t)ni, gn - ghead);
                 {
                     const bool s1 = idle && rank < t1;
