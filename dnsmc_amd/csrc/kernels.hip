@@ -416,10 +416,11 @@ __device__ __forceinline__ void tally_ray(const TraceArgs& P, float er, float ar
 
 // A drawn ray, as the refill hands it to a lane: bricked index and packed
 // (i, j, k) of the emitting cell, its tally slot, band | g << 16 | sign flags
-// << 24, emitter temperature, direction and its reciprocals, weights.
+// << 24, its processing slot in the launch, emitter temperature, direction and
+// its reciprocals, weights, emission point (the cell centre).
 struct RayStart {
-    int idx, cell, six, bg;
-    float Tc, sx, sy, sz, ix, iy, iz, rw, rwi;
+    int idx, cell, six, bg, ps;
+    float Tc, sx, sy, sz, ix, iy, iz, rw, rwi, px, py, pz;
 };
 
 // Lane `src`'s ray (all 64 lanes call it).
@@ -430,6 +431,7 @@ __device__ __forceinline__ RayStart pull_ray(const RayStart& r, int src)
     o.cell = __shfl(r.cell, src);
     o.six = __shfl(r.six, src);
     o.bg = __shfl(r.bg, src);
+    o.ps = __shfl(r.ps, src);
     o.Tc = __shfl(r.Tc, src);
     o.sx = __shfl(r.sx, src);
     o.sy = __shfl(r.sy, src);
@@ -439,6 +441,9 @@ __device__ __forceinline__ RayStart pull_ray(const RayStart& r, int src)
     o.iz = __shfl(r.iz, src);
     o.rw = __shfl(r.rw, src);
     o.rwi = __shfl(r.rwi, src);
+    o.px = __shfl(r.px, src);
+    o.py = __shfl(r.py, src);
+    o.pz = __shfl(r.pz, src);
     return o;
 }
 
@@ -479,11 +484,19 @@ template <int SPEC, int TAB, int TMODE, bool REC, bool AMC, bool GEN>
 __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArgs P)
 {
     extern __shared__ float lds[];
-    int* s_lev = (int*)lds;                     // 6 * kLevWords
-    float* s_face = lds + 6 * kLevWords;
+    // the wave's tally ring (atomic tallies, no records), first in LDS:
+    // kRing entries of FP64 (E, A, sum x^2), the processing slot each holds
+    // (-1: free) and its tally slot
+    constexpr bool QUEUE = TMODE == 0 && !REC;
+    const int nwv = QUEUE ? (int)(blockDim.x >> 6) : 0;
+    const int rb = (int)(threadIdx.x >> 6) * kRing;   // this wave's first entry
+    double* s_rv = (double*)lds;
+    int* s_rt = (int*)(s_rv + 3 * kRing * nwv);
+    int* s_rg = s_rt + kRing * nwv;
+    int* s_lev = s_rg + kRing * nwv;            // 6 * kLevWords
+    float* s_face = (float*)(s_lev + 6 * kLevWords);
     int* s_off = (int*)(s_face + P.nfaces);     // brick offsets per level and axis
-    float* s_cent = (float*)(s_off + P.noffs);  // level-0 cell centres (ghost-inclusive)
-    float* s_ibw = s_cent + P.ncent;            // 6 * nB wall Planck values
+    float* s_ibw = (float*)(s_off + P.noffs);   // 6 * nB wall Planck values
     float* s_ib = s_ibw + 6 * P.nB;             // Ib rows (nT or tab_nw nodes)
     const int nw = P.tab_nw;
     float* s_kq = s_ib + P.nB * nw;             // sliced kappa rows (kTabSliced)
@@ -509,8 +522,13 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     }
     for (int i = threadIdx.x; i < P.nfaces; i += blockDim.x) s_face[i] = P.faces[i];
     for (int i = threadIdx.x; i < P.noffs; i += blockDim.x) s_off[i] = P.offs[i];
-    for (int i = threadIdx.x; i < P.ncent; i += blockDim.x) s_cent[i] = P.centres[i];
     for (int i = threadIdx.x; i < 6 * P.nB; i += blockDim.x) s_ibw[i] = P.Ibw[i];
+    if (QUEUE && (threadIdx.x & 63) < kRing) {
+        const int e = rb + (int)(threadIdx.x & 63);
+        s_rt[e] = -1;
+        s_rg[e] = 0;
+        s_rv[3 * e] = s_rv[3 * e + 1] = s_rv[3 * e + 2] = 0.0;
+    }
     if (GEN) {
         for (int i = threadIdx.x; i < P.nB; i += blockDim.x) s_cdfb[i] = P.cdf_band[i];
         if (P.gen_lds)
@@ -573,8 +591,31 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     // the wave's queue of drawn rays (lane j: ray gbase + j, j < gn); the
     // per-ray outputs (TMODE 1) and records index rays by lane, so they keep
     // the direct hand-out
-    constexpr bool QUEUE = TMODE == 0 && !REC;
     uint32_t gn = 0, ghead = 0, gsnap = 0;
+    int pslot = 0;                        // processing slot of the lane's ray (tally ring tag)
+    const int nslot_l = (int)(nrays / (uint32_t)P.rays_per_slot);   // processing slots per snapshot
+    // the finished ray's (e, a): into its ring entry while the entry holds its
+    // slot, else (evicted) straight to the tally
+    auto ring_tally = [&](float er, float ar) {
+        const double de = (double)er, da = (double)ar, x = de - da;
+        const int e = rb + (pslot & (kRing - 1));
+        if (s_rt[e] == pslot) {
+            __hip_atomic_fetch_add(&s_rv[3 * e], de, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&s_rv[3 * e + 1], da, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&s_rv[3 * e + 2], x * x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            unsafeAtomicAdd(&P.E[rslot], de);
+            unsafeAtomicAdd(&P.A[rslot], da);
+            unsafeAtomicAdd(&P.Q2[rslot], x * x);
+        }
+    };
+    // ring entry e's sums to its tally slot (one lane)
+    auto ring_flush = [&](int e) {
+        const int t = s_rg[e];
+        unsafeAtomicAdd(&P.E[t], s_rv[3 * e]);
+        unsafeAtomicAdd(&P.A[t], s_rv[3 * e + 1]);
+        unsafeAtomicAdd(&P.Q2[t], s_rv[3 * e + 2]);
+    };
     RayStart Qr = {};
 
     // ray q of snapshot qs of the launch: drawn here (GEN, the plain estimator's
@@ -586,6 +627,10 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         rd.six = si.x;
         rd.idx = si.z;
         rd.cell = si.w;
+        rd.ps = (int)slot + (int)qs * nslot_l;
+        rd.px = P.centres[P.coff[0] + (si.w & 1023) + 1];
+        rd.py = P.centres[P.coff[1] + ((si.w >> 10) & 1023) + 1];
+        rd.pz = P.centres[P.coff[2] + ((si.w >> 20) & 1023) + 1];
         gfloat* Te = lev_T(s_lev) + (GEN ? (size_t)qs * (uint32_t)s_lev[14] : (size_t)0);
         const float Tq = Te[si.z];
         rd.Tc = Tq;
@@ -680,9 +725,12 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         if (na == 0 && !have_rays) break;
         if (have_rays && (ni >= P.refill_k || na == 0)) {
             // ---- tally the rays finished since the last refill
-            if (TMODE == 0) flush_wave(P, fin && rslot != aslot && aslot >= 0, aslot, aE, aA, aQ, lane);
+            if (!QUEUE && TMODE == 0) flush_wave(P, fin && rslot != aslot && aslot >= 0, aslot, aE, aA, aQ, lane);
             if (fin) {
-                tally_ray<TMODE>(P, rw * Dfin, rwi * G, rslot, qcur, aslot, aE, aA, aQ);
+                if (QUEUE)
+                    ring_tally(rw * Dfin, rwi * G);
+                else
+                    tally_ray<TMODE>(P, rw * Dfin, rwi * G, rslot, qcur, aslot, aE, aA, aQ);
                 if (REC && qcur < P.nrec) {
                     P.rec_steps[qcur] = rsteps;
                     P.rec_end[qcur] = endt;
@@ -710,9 +758,10 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 cx = (rd.cell & 1023) + 1;
                 cy = ((rd.cell >> 10) & 1023) + 1;
                 cz = ((rd.cell >> 20) & 1023) + 1;
-                px = s_cent[P.coff[0] + cx];
-                py = s_cent[P.coff[1] + cy];
-                pz = s_cent[P.coff[2] + cz];
+                px = rd.px;
+                py = rd.py;
+                pz = rd.pz;
+                pslot = rd.ps;
                 if (AMC && lev != 0) {
                     lev = 0;
                     n0 = s_lev[0]; n1 = s_lev[1]; n2 = s_lev[2];
@@ -779,6 +828,24 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                         gsnap = csnap;
                         if ((uint32_t)lane < gn) Qr = draw_ray(cur + (uint32_t)lane, gsnap);
                         cur += gn;
+                        // ring entries for the batch's processing slots (consecutive:
+                        // lane 0's to lane gn-1's); an entry still holding an older
+                        // slot hands its sums over first (its rays still in flight
+                        // then tally straight to the tally slot)
+                        const int ps0 = __shfl(Qr.ps, 0), ps1 = __shfl(Qr.ps, (int)gn - 1);
+                        for (int sp = ps0; sp <= ps1; ++sp) {
+                            const int e = rb + (sp & (kRing - 1));
+                            const int held = s_rt[e];
+                            if (held == sp) continue;
+                            const unsigned long long mine = __ballot((uint32_t)lane < gn && Qr.ps == sp);
+                            const int tgt = __shfl(Qr.six, __ffsll((long long)mine) - 1) + (int)gsnap * P.tally_stride;
+                            if (lane == 0) {
+                                if (held >= 0) ring_flush(e);
+                                s_rt[e] = sp;
+                                s_rg[e] = tgt;
+                                s_rv[3 * e] = s_rv[3 * e + 1] = s_rv[3 * e + 2] = 0.0;
+                            }
+                        }
                         const bool s2 = idle && rank >= t1 && rank - t1 < gn;
                         const RayStart rd = pull_ray(Qr, (int)(s2 ? rank - t1 : 0u));
                         if (s2) start_ray(rd, gsnap);
@@ -955,9 +1022,12 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         }
     }
     // a ray that ended after the last refill (nothing was left to hand out)
-    if (TMODE == 0) flush_wave(P, fin && rslot != aslot && aslot >= 0, aslot, aE, aA, aQ, lane);
+    if (!QUEUE && TMODE == 0) flush_wave(P, fin && rslot != aslot && aslot >= 0, aslot, aE, aA, aQ, lane);
     if (fin) {
-        tally_ray<TMODE>(P, rw * Dfin, rwi * G, rslot, qcur, aslot, aE, aA, aQ);
+        if (QUEUE)
+            ring_tally(rw * Dfin, rwi * G);
+        else
+            tally_ray<TMODE>(P, rw * Dfin, rwi * G, rslot, qcur, aslot, aE, aA, aQ);
         if (REC && qcur < P.nrec) {
             P.rec_steps[qcur] = rsteps;
             P.rec_end[qcur] = endt;
@@ -967,7 +1037,9 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         }
         steps_lane += (uint32_t)rsteps;
     }
-    if (TMODE == 0) flush_wave(P, aslot >= 0, aslot, aE, aA, aQ, lane);
+    if (!QUEUE && TMODE == 0) flush_wave(P, aslot >= 0, aslot, aE, aA, aQ, lane);
+    // the ring's remaining sums
+    if (QUEUE && lane < kRing && s_rt[rb + lane] >= 0) ring_flush(rb + lane);
     // one step-count atomic per wave
     unsigned long long st = steps_lane;
     for (int off = 32; off > 0; off >>= 1) st += __shfl_xor(st, off);

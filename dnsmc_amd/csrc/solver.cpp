@@ -747,8 +747,12 @@ static int do_run(dnsmc_solver* s)
     int tab = kTabGlobal;
     P.tab_lo = 0;
     P.tab_nw = d.nT;
-    size_t lds_trace = sizeof(float) * (6 * kLevWords + P.nfaces + P.noffs + P.ncent + 6 * (size_t)d.nB +
-                                        (fused ? (size_t)d.nB : 0));
+    // the atomic-tally launches keep a tally ring per wave first in LDS (FP64
+    // E, A, sum x^2 + two ints per entry, room for 16 waves); the cell centres
+    // are read at ray start from global memory
+    const size_t ring_bytes = (!det && !s->record) ? (size_t)16 * kRing * (3 * sizeof(double) + 2 * sizeof(int)) : 0;
+    size_t lds_trace = ring_bytes + sizeof(float) * (6 * kLevWords + P.nfaces + P.noffs + 6 * (size_t)d.nB +
+                                                     (fused ? (size_t)d.nB : 0));
     if (spec == kSpecTable) {
         float tlo = s->Tlo_snap[0], thi = s->Thi_snap[0];
         for (int i = 1; i < c.nsnap; ++i) {
