@@ -795,11 +795,10 @@ static int do_run(dnsmc_solver* s)
     P.fd_R = make_fastdiv((uint32_t)R);
     {
         // measured defaults: the fused and snapshot-wise launches refill at 24
-        // idle lanes (40 with coarsening levels, whose rays are short, so the
-        // refill weighs more) and hand out 512-ray super-chunks; the batched
-        // launches 16 and 256
+        // idle lanes and hand out 512-ray super-chunks; the batched launches
+        // 16 and 256
         const char* rk = getenv("DNSMC_REFILL_K");
-        P.refill_k = rk ? atoi(rk) : (fused || snapwise ? (P.nlev > 1 ? 40 : 24) : 16);
+        P.refill_k = rk ? atoi(rk) : (fused || snapwise ? 24 : 16);
         if (P.refill_k < 1) P.refill_k = 1;
         if (P.refill_k > 64) P.refill_k = 64;
         const char* sc = getenv("DNSMC_SUPER");
