@@ -94,7 +94,7 @@ struct TraceArgs {
     const float* inv_emax;
     float Ibmax;
     int g_search;
-    int gen_lds;                // cdf_g and inv_emax staged in LDS too
+    int gen_lds;                // bit 0: cdf_g staged in LDS too; bit 1: inv_emax too
     int desc_ratio;             // learned descriptors: w holds p/q, the weight is computed at ray start
     // FP64 traversal extras
     const double* inv_emax_d;

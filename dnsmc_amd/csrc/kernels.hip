@@ -502,7 +502,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     float* s_kq = s_ib + P.nB * nw;             // sliced kappa rows (kTabSliced)
     // GEN: band CDF after the tables that are in LDS
     float* s_cdfb = TAB == kTabSliced ? s_kq + P.nB * P.nQ * nw : (TAB == kTabIbLds ? s_ib + P.nB * P.nT : s_ib);
-    float* s_cdfg = s_cdfb + P.nB;              // GEN with gen_lds: g CDF rows and 1/(k Ib)(Tmax)
+    float* s_cdfg = s_cdfb + P.nB;              // GEN, gen_lds bit 0: g CDF rows; bit 1: 1/(k Ib)(Tmax)
     float* s_iem = s_cdfg + P.nB * P.nQ;
     for (int i = threadIdx.x; i < P.nlev; i += blockDim.x) {
         int* L = s_lev + i * kLevWords;
@@ -531,11 +531,10 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     }
     if (GEN) {
         for (int i = threadIdx.x; i < P.nB; i += blockDim.x) s_cdfb[i] = P.cdf_band[i];
-        if (P.gen_lds)
-            for (int i = threadIdx.x; i < P.nB * P.nQ; i += blockDim.x) {
-                s_cdfg[i] = P.cdf_g[i];
-                s_iem[i] = P.inv_emax[i];
-            }
+        if (P.gen_lds & 1)
+            for (int i = threadIdx.x; i < P.nB * P.nQ; i += blockDim.x) s_cdfg[i] = P.cdf_g[i];
+        if (P.gen_lds & 2)
+            for (int i = threadIdx.x; i < P.nB * P.nQ; i += blockDim.x) s_iem[i] = P.inv_emax[i];
     }
     if (SPEC == kSpecTable && TAB == kTabIbLds)
         for (int i = threadIdx.x; i < P.nB * P.nT; i += blockDim.x) s_ib[i] = P.Ib[i];
@@ -645,7 +644,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
             bnd = cdf_search(s_cdfb, P.nB, u01_open(x.x));
             if (!P.g_search)
                 g = 0;
-            else if (P.gen_lds)
+            else if (P.gen_lds & 1)
                 g = cdf_search(s_cdfg + bnd * P.nQ, P.nQ, u01_open(x.y));
             else
                 g = cdf_search(P.cdf_g + bnd * P.nQ, P.nQ, u01_open(x.y));
@@ -688,7 +687,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 kap = fmaf(fr, k1 - k0, k0);
                 ib = fmaf(fr, i1 - i0, i0);
             }
-            const float iem = P.gen_lds ? s_iem[bnd * P.nQ + g] : P.inv_emax[bnd * P.nQ + g];
+            const float iem = (P.gen_lds & 2) ? s_iem[bnd * P.nQ + g] : P.inv_emax[bnd * P.nQ + g];
             const float w = P.Ibmax * (kap * ib) * iem;
             const float wi = w * (1.0f / ib);
             rd.rw = GEN ? w : w * ratio;

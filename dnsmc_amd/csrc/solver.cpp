@@ -780,15 +780,19 @@ static int do_run(dnsmc_solver* s)
             lds_trace += sizeof(float) * (size_t)d.nB * d.nT;
         }
     }
-    // fused generation: the g CDF rows and 1/(k Ib)(Tmax) join the band CDF in
-    // LDS when the block's share still holds them
+    // fused generation: the g CDF rows, then 1/(k Ib)(Tmax), join the band CDF
+    // in LDS when the block's share still holds them
     P.gen_lds = 0;
     P.desc_ratio = c.mode == DNSMC_MODE_LEARNED ? 1 : 0;
     P.inv_emax = s->d_inv_emax.p;
     P.Ibmax = d.Ibmax;
-    if (fused && spec == kSpecTable && lds_trace + 2 * sizeof(float) * (size_t)d.nB * d.nQ <= s->lds_max) {
+    const size_t gtab = sizeof(float) * (size_t)d.nB * d.nQ;
+    if (fused && spec == kSpecTable && lds_trace + 2 * gtab <= s->lds_max) {
+        P.gen_lds = 3;
+        lds_trace += 2 * gtab;
+    } else if (fused && spec == kSpecTable && lds_trace + gtab <= s->lds_max) {
         P.gen_lds = 1;
-        lds_trace += 2 * sizeof(float) * (size_t)d.nB * d.nQ;
+        lds_trace += gtab;
     }
     P.rays_per_slot = R;
     P.slots_per_emitter = spe;
