@@ -674,13 +674,21 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 kap = P.kappa_grey;
                 ib = t2 * t2;
             } else {
-                float pos = fmaf(Tq, P.inv_dT, P.mT0);
-                pos = fminf(fmaxf(pos, 0.0f), (float)(nT - 1));
-                int t0 = (int)pos;
-                t0 = t0 > nT - 2 ? nT - 2 : t0;
-                const float fr = pos - (float)t0;
+                // sliced tables: Tq is the table coordinate within the slice
+                int t;
+                float fr;
+                if (TAB == kTabSliced) {
+                    t = (int)Tq;
+                    t = t > nw - 2 ? nw - 2 : t;
+                    fr = Tq - (float)t;
+                } else {
+                    float pos = fmaf(Tq, P.inv_dT, P.mT0);
+                    pos = fminf(fmaxf(pos, 0.0f), (float)(nT - 1));
+                    t = (int)pos;
+                    t = t > nT - 2 ? nT - 2 : t;
+                    fr = pos - (float)t;
+                }
                 const int kr = (bnd * P.nQ + g) * rowlen, ir = bnd * rowlen;
-                const int t = TAB == kTabSliced ? max(0, min(t0 - P.tab_lo, nw - 2)) : t0;
                 const float k0 = TAB == kTabSliced ? s_kq[kr + t] : kq[kr + t];
                 const float k1 = TAB == kTabSliced ? s_kq[kr + t + 1] : kq[kr + t + 1];
                 const float i0 = ibt[ir + t], i1 = ibt[ir + t + 1];
@@ -906,20 +914,22 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         // ---------------------------------------------------- one crossing
         float k0 = 0.f, k1 = 0.f, i0 = 0.f, i1 = 0.f, fr = 0.f;
         if (SPEC == kSpecTable) {
-            float pos = fmaf(Tc, P.inv_dT, P.mT0);
-            pos = fminf(fmaxf(pos, 0.0f), (float)(nT - 1));
-            int t0 = (int)pos;
-            t0 = t0 > nT - 2 ? nT - 2 : t0;
-            fr = pos - (float)t0;
             if (TAB == kTabSliced) {
-                // node t0 of the slice; the slice spans every temperature of the
-                // fields, the clamp only keeps a corrupt field inside the table
-                const int w = max(0, min(t0 - P.tab_lo, nw - 2));
+                // Tc is the cell's table coordinate within the slice (t_to_u):
+                // node w of the slice and the lerp fraction
+                int w = (int)Tc;
+                w = w > nw - 2 ? nw - 2 : w;
+                fr = Tc - (float)w;
                 k0 = s_kq[krow + w];
                 k1 = s_kq[krow + w + 1];
                 i0 = s_ib[ibrow + w];
                 i1 = s_ib[ibrow + w + 1];
             } else {
+                float pos = fmaf(Tc, P.inv_dT, P.mT0);
+                pos = fminf(fmaxf(pos, 0.0f), (float)(nT - 1));
+                int t0 = (int)pos;
+                t0 = t0 > nT - 2 ? nT - 2 : t0;
+                fr = pos - (float)t0;
                 k0 = kq[krow + t0];
                 k1 = kq[krow + t0 + 1];
                 i0 = ibt[ibrow + t0];
@@ -1441,6 +1451,19 @@ __global__ void field_to_slab(const int* emit, long long nemit, const double* Q,
     const long long o = (long long)(i + 1) + (long long)(n0 + 2) * (jl + (long long)nj * kl);
     divq[o] = Q[e] / stefan / tc4;
     vcore[o] = sd[e] / stefan / tc4;
+}
+
+// Table-coordinate field of the sliced-table traversal: u = clamp(T/dT - T0/dT,
+// 0, nT-1) - lo, the position of each cell's temperature among the nodes held
+// in LDS.  The traversal's lerp from u is bit-identical to the one from T (the
+// subtraction of the integer lo from the clamped position is exact).
+__global__ void t_to_u(const float* T, float* U, long long n, float inv_dT, float mT0, float top, float lo)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        float pos = fmaf(T[i], inv_dT, mT0);
+        pos = fminf(fmaxf(pos, 0.0f), top);
+        U[i] = pos - lo;
+    }
 }
 
 __global__ void fill_u32(unsigned* p, long long n, unsigned v)

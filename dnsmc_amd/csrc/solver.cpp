@@ -55,6 +55,7 @@ __global__ void mlp_samples(FeatArgs f, const int4* slotinfo, long long slot0, i
                             int* tb, float* wt);
 __global__ void mlp_features(FeatArgs f, const int4* slotinfo, long long n, float* X);
 __global__ void fill_u32(unsigned* p, long long n, unsigned v);
+__global__ void t_to_u(const float* T, float* U, long long n, float inv_dT, float mT0, float top, float lo);
 __global__ void slab_to_field(const double* cnew, int n0, int n1, int n2, int i0, int j0, int k0, int ni, int nj,
                               int nk, double Tc, double Tplus, float* T, int nbx, int nby);
 __global__ void field_to_slab(const int* emit, long long nemit, const double* Q, const double* sd, int n0, int n1,
@@ -131,6 +132,7 @@ struct dnsmc_solver {
     DevBuf<unsigned> d_lohi;
     DevBuf<double> d_slab;         // staging of the DNS coupling slabs
     DevBuf<float> d_T[kMaxLev];
+    DevBuf<float> d_U[kMaxLev];    // table coordinates of the sliced-table traversal
     long long ncell[kMaxLev] = {0, 0, 0, 0, 0, 0};     // cells per level
     long long tstride[kMaxLev] = {0, 0, 0, 0, 0, 0};   // bricked floats per level and snapshot
     int nbx[kMaxLev] = {0, 0, 0, 0, 0, 0}, nby[kMaxLev] = {0, 0, 0, 0, 0, 0};
@@ -775,6 +777,22 @@ static int do_run(dnsmc_solver* s)
             P.tab_lo = lo;
             P.tab_nw = nw;
             lds_trace = sliced;
+            if (!f64) {
+                // the FP32 traversal reads each cell's table coordinate within the
+                // slice instead of its temperature (t_to_u), every level and snapshot
+                for (size_t l = 0; l < s->lev.size(); ++l) {
+                    const long long nl = s->tstride[l] * c.nsnap;
+                    HIPCHK(s->d_U[l].ensure(nl));
+                    const int nb = (int)std::min<long long>((nl + 255) / 256, (long long)s->ncu * 16);
+                    hipLaunchKernelGGL(t_to_u, dim3(nb), dim3(256), 0, s->stream, (const float*)s->d_T[l].p,
+                                       s->d_U[l].p, nl, d.inv_dT, d.mT0, (float)(d.nT - 1), (float)lo);
+                    HIPCHK(hipGetLastError());
+                    P.T[l] = s->d_U[l].p;
+                }
+                // the pipeline streams start after the coordinates are written
+                HIPCHK(hipEventRecord(s->ev_ready, s->stream));
+                for (int p = 1; p < npipes; ++p) HIPCHK(hipStreamWaitEvent(s->pipe[p].stream, s->ev_ready, 0));
+            }
         } else if (allow && (size_t)d.nB * d.nT <= 12288) {
             tab = kTabIbLds;
             lds_trace += sizeof(float) * (size_t)d.nB * d.nT;
