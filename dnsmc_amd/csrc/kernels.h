@@ -9,7 +9,7 @@
 namespace dn {
 
 constexpr int kLevWords = 16;   // per-level parameter words staged in LDS
-constexpr int kRing = 4;        // tally ring entries per wave (power of 2)
+constexpr int kRing = 8;        // tally ring entries per wave (power of 2)
 
 // Per-level geometry, staged into LDS as ints (floats for nothing).
 struct LevParams {
