@@ -240,6 +240,32 @@ def test_tally_modes_agree(tally):
     np.testing.assert_allclose(out[1]["A"], out[0]["A"], rtol=tol)
 
 
+@pytest.mark.parametrize("R", [2, 3, 8, 24])
+def test_tally_ring_with_few_rays_per_cell(ck_table_dir, R):
+    """The per-wave tally ring (atomic tallies) against the deterministic
+    per-ray FP64 sums when a 64-ray queue batch spans many processing slots
+    (R = 2: 32 slots, more than the ring holds, so entries are evicted while
+    their rays are in flight and those rays tally straight to the slot): E,
+    A and the variance sums agree to FP64 rounding, on the fine grid and with
+    coarsening levels."""
+    tdir, _ = ck_table_dir
+    for amc in ({}, dict(grid_num=3, maxs=(3, 3, 100000))):
+        out = []
+        for tally in (0, 1):
+            cfg = small_channel(n=(12, 16, 8), R=R, spectral=0, Tw=573.0, table_dir=tdir, tally=tally, **amc)
+            S = dnsmc_amd.Solver(cfg)
+            S.load_tables()
+            S.set_field(0, workloads.ghosted(workloads.field_profile_noise(7, tuple(cfg.n))))
+            S.run()
+            out.append((S.results(0), S.stats()))
+            S.close()
+        (r0, s0), (r1, s1) = out
+        assert s0["n_steps"] == s1["n_steps"]
+        np.testing.assert_allclose(r0["E"], r1["E"], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(r0["A"], r1["A"], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(r0["sdev"], r1["sdev"], rtol=1e-9, atol=1e-300)
+
+
 def test_table_placement_bit_identical(ck_table_dir, monkeypatch):
     """The kappa/Ib slice in LDS and the global-memory tables give the same
     per-ray values, so the deterministic FP64 tallies agree exactly."""
