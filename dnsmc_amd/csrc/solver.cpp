@@ -816,11 +816,11 @@ static int do_run(dnsmc_solver* s)
     P.slots_per_emitter = spe;
     P.fd_R = make_fastdiv((uint32_t)R);
     {
-        // measured defaults: the fused and snapshot-wise launches refill at 24
-        // idle lanes and hand out 512-ray super-chunks; the batched launches
-        // 16 and 256
+        // measured defaults: launches with the ray queue (atomic tallies, no
+        // records) refill at 24 idle lanes, the others at 16; the fused and
+        // snapshot-wise launches hand out 512-ray super-chunks, the batched 256
         const char* rk = getenv("DNSMC_REFILL_K");
-        P.refill_k = rk ? atoi(rk) : (fused || snapwise ? 24 : 16);
+        P.refill_k = rk ? atoi(rk) : (!det && !s->record ? 24 : 16);
         if (P.refill_k < 1) P.refill_k = 1;
         if (P.refill_k > 64) P.refill_k = 64;
         const char* sc = getenv("DNSMC_SUPER");
