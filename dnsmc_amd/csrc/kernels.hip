@@ -461,23 +461,30 @@ __device__ __forceinline__ RayStart pull_ray(const RayStart& r, int src)
 //
 // Lanes whose ray ended stay idle until `refill_k` lanes of the wave are idle
 // (or none is alive); one wave-uniform refill block then tallies the finished
-// rays and starts new rays on every idle lane (consecutive rays on
-// consecutive lanes: coalesced descriptor loads).
+// rays and starts new rays on every idle lane, in lane order.  With atomic
+// tallies and no records (QUEUE) the new rays come from the wave's queue of 64
+// drawn rays (lane j holds ray gbase + j), redrawn 64 at a time on the whole
+// wave; otherwise each idle lane draws its own ray.
 //
-// Tallies: TMODE 0 keeps a running FP64 (E, A, sum x^2) per lane for its
-// current tally slot and adds it to the slot with three FP64 atomics when the
-// lane moves to a ray of another slot (consecutive rays share a slot, so one
-// atomic set covers several rays).  TMODE 1 writes each ray's (E, A) for the
-// deterministic reduce_rays pass.
+// Tallies: QUEUE keeps a ring of kRing slot accumulators per wave in LDS
+// (FP64 E, A, sum x^2): the processing slots of a drawn queue get entries
+// (slot mod kRing; an entry still holding an older slot first adds its sums
+// to that slot with three device FP64 atomics), and a finished ray adds to
+// its entry with LDS atomics while the entry holds its slot, else straight to
+// the slot.  TMODE 0 with records keeps a running FP64 sum per lane, handed
+// over with wave-aggregated atomics (flush_wave) when the lane moves to
+// another slot.  TMODE 1 writes each ray's (E, A) for the deterministic
+// reduce_rays pass.
 //
 // One step: the kappa gather of the current cell is issued first, the
 // geometry (distances, face snap, move, boundary) runs while it is in flight,
-// the next cell's temperature load is issued at the end of the geometry, and
-// the optical depth / transmissivity / tally update consumes kappa last.
+// the next cell's field load is issued at the end of the geometry, and the
+// optical depth / transmissivity / tally update consumes kappa last.  With
+// the kappa/Ib slice in LDS the field holds table coordinates (t_to_u).
 //
-// GEN: the rays are drawn in the refill itself (the plain estimator's Philox
-// draw, bit-identical to raygen_plain) instead of read from descriptors, and
-// one launch covers every snapshot: XCD x's claims walk snapshot by snapshot
+// GEN: the rays are drawn in the kernel (the plain estimator's Philox draw,
+// bit-identical to raygen_plain) instead of read from descriptors, and one
+// launch covers every snapshot: XCD x's claims walk snapshot by snapshot
 // through its 1/8 of each snapshot's super-chunks, so all XCDs work on the
 // same snapshot at a time.
 template <int SPEC, int TAB, int TMODE, bool REC, bool AMC, bool GEN>
