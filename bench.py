@@ -183,10 +183,13 @@ def learned_leg(args, cfg, R, fields, res_p, t_plain, cfg_kw):
             S1, _ = study.make_solver(args.workload, RR, mode=mode, mlp=W, mix_eps=args.mix_eps, fields=f0, **cfg_kw)
             S1.run()
             S1.wait()
-            t0 = time.perf_counter()
-            S1.run()
-            S1.wait()
-            ts.append(time.perf_counter() - t0)
+            reps = []
+            for _ in range(3):   # median of three solves
+                t0 = time.perf_counter()
+                S1.run()
+                S1.wait()
+                reps.append(time.perf_counter() - t0)
+            ts.append(float(np.median(reps)))
             S1.close()
         b = (ts[1] - ts[0]) / (3 * R)
         a = ts[0] - b * R
