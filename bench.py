@@ -204,7 +204,7 @@ def learned_leg(args, cfg, R, fields, res_p, t_plain, cfg_kw):
                                learned={str(R): lin["learned"][2][0], str(4 * R): lin["learned"][2][1]}),
         at_bench_rays=dict(plain=t_p, learned=t_l, speedup_learned=t_p / t_l),
         note="rays per cell to 1%% RMS from the error at %d rays/cell (error ~ R^-1/2); solve time at that "
-             "R from the measured fixed + per-ray cost on snapshot 0 (R and 4R) times %d snapshots; "
+             "R from the measured fixed + per-ray cost on snapshot 0 (R and 4R, median of three solves each) times %d snapshots; "
              "at_bench_rays scales the whole measured solve linearly instead" % (R, ns))
     return out
 
