@@ -1014,7 +1014,6 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         } else if (cap) {
             if (AMC && lev + 1 < P.nlev) {
                 // adaptive mesh coarsening (code/mc_gpu.cu:193-207)
-                const int o0 = n0, o1 = n1, o2 = n2;
                 ++lev;
                 const int* L = s_lev + lev * kLevWords;
                 n0 = L[0]; n1 = L[1]; n2 = L[2];
@@ -1022,10 +1021,11 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 oo0 = L[11]; oo1 = L[12]; oo2 = L[13];
                 Tl = lev_T(L);
                 if (GEN) Tl += (size_t)rsnap * (uint32_t)L[14];
-                const int q0 = o0 / n0, q1 = o1 / n1, q2 = o2 / n2;
-                cx = (cx + q0 - 1) / q0;
-                cy = (cy + q1 - 1) / q1;
-                cz = (cz + q2 - 1) / q2;
+                // ic = (ic + r - 1) / r with r = 2: every level halves each axis
+                // (level g has n >> g cells)
+                cx = (cx + 1) >> 1;
+                cy = (cy + 1) >> 1;
+                cz = (cz + 1) >> 1;
                 fx = s_face[fo0 + cx - (fl & 1)];
                 fy = s_face[fo1 + cy - ((fl >> 1) & 1)];
                 fz = s_face[fo2 + cz - ((fl >> 2) & 1)];
