@@ -635,9 +635,10 @@ static int do_run(dnsmc_solver* s)
         slots_per_batch = std::max<long long>(1, std::min<long long>(slots_per_batch,
                                                                      (8ll << 30) / (4ll * mlp_noutp(s->mlp_nbin))));
     // plain estimator with atomic tallies and no records: rays are drawn inside
-    // the traversal, one launch for all snapshots (DNSMC_FUSED=0: batched)
-    bool fused = c.mode == DNSMC_MODE_PLAIN && c.precision == 0 && c.tally == DNSMC_TALLY_F64_ATOMIC && !s->record &&
-                 nslots * (long long)R <= (1ll << 32) - (1ll << 20);
+    // the traversal, one launch for all snapshots (DNSMC_FUSED=0: batched);
+    // more than 2^32 - 2^20 rays per snapshot take several launches over
+    // consecutive slot ranges (ray indices within a launch are 32-bit)
+    bool fused = c.mode == DNSMC_MODE_PLAIN && c.precision == 0 && c.tally == DNSMC_TALLY_F64_ATOMIC && !s->record;
     if (const char* fe = getenv("DNSMC_FUSED")) fused = fused && atoi(fe) != 0;
     if (fused) slots_per_batch = nslots;
     // learned estimator, atomic tallies, no records: the descriptors of a whole
@@ -899,8 +900,6 @@ static int do_run(dnsmc_solver* s)
         hipStream_t st = pp.stream;
         P.nsnap_launch = c.nsnap;
         P.tally_stride = (int)nslots;
-        P.nrays = nslots * R;
-        P.slot0 = 0;
         P.snap = 0;
         P.seed = c.seed;
         P.cdf_band = s->d_cdfb.p;
@@ -915,15 +914,21 @@ static int do_run(dnsmc_solver* s)
         P.Q2 = s->d_Q2.p;
         P.nrec = 0;
         P.claim = pp.d_claim.p;
-        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, pp.d_claim.p, 256ll, 0u);
-        HIPCHK(hipGetLastError());
-        const long long nsc = ((P.nrays + P.super - 1) / P.super) * c.nsnap;
-        const int tb_cap = (int)std::min<long long>(trace_blocks, std::max<long long>(8, (nsc + 15) / 16));
-        EventPair& et = next_event(s, 1);
-        HIPCHK(hipEventRecord(et.a, st));
-        HIPCHK(launch_trace(spec, tab, true, false, false, amc, s->ncu, tb_cap, lds_trace, st, P));
-        HIPCHK(hipEventRecord(et.b, st));
-        s->st.n_trace_launches++;
+        long long part = std::max<long long>(1, ((1ll << 32) - (1ll << 20)) / R);   // slots per launch
+        if (const char* fp = getenv("DNSMC_FUSED_SLOTS")) part = std::max<long long>(1, std::min(part, atoll(fp)));
+        for (long long p0 = 0; p0 < nslots; p0 += part) {
+            P.slot0 = p0;
+            P.nrays = std::min(part, nslots - p0) * R;
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, pp.d_claim.p, 256ll, 0u);
+            HIPCHK(hipGetLastError());
+            const long long nsc = ((P.nrays + P.super - 1) / P.super) * c.nsnap;
+            const int tb_cap = (int)std::min<long long>(trace_blocks, std::max<long long>(8, (nsc + 15) / 16));
+            EventPair& et = next_event(s, 1);
+            HIPCHK(hipEventRecord(et.a, st));
+            HIPCHK(launch_trace(spec, tab, true, false, false, amc, s->ncu, tb_cap, lds_trace, st, P));
+            HIPCHK(hipEventRecord(et.b, st));
+            s->st.n_trace_launches++;
+        }
     }
     if (snapwise) {
         // generation on pipe 1 into descriptor set (snap % 2), one traversal

@@ -375,13 +375,17 @@ def test_fp64_traversal_matches_fp64_oracle(oracle, ck_table_dir, spectral):
     assert np.max(np.abs(r32["Q"] - (E - A)) / np.abs(E)) < 1e-5
 
 
-@pytest.mark.parametrize("spectral,grid_num", [(0, 1), (0, 2), (3, 1)])
-def test_fused_generation_matches_batched(ck_table_dir, monkeypatch, spectral, grid_num):
+@pytest.mark.parametrize("spectral,grid_num,part", [(0, 1, None), (0, 2, None), (3, 1, None), (0, 2, 37)])
+def test_fused_generation_matches_batched(ck_table_dir, monkeypatch, spectral, grid_num, part):
     """The plain estimator's rays drawn inside the traversal (one launch for
     all snapshots) against descriptors drawn by raygen_plain (batched
     launches, which the oracle pins ray by ray): same rays, so equal step
-    totals and tallies equal up to the order of the FP64 atomic additions."""
+    totals and tallies equal up to the order of the FP64 atomic additions.
+    part: slots per fused launch (the split a snapshot of more than 2^32
+    rays takes), here 37 of the 4096 slots per snapshot."""
     tdir, _ = ck_table_dir
+    if part is not None:
+        monkeypatch.setenv("DNSMC_FUSED_SLOTS", str(part))
     n, R, ns = (16, 24, 16), 48, 3
     maxs = (6, 100000, 100000, 100000, 100000, 100000)
     out = {}
@@ -400,7 +404,8 @@ def test_fused_generation_matches_batched(ck_table_dir, monkeypatch, spectral, g
         out[fused] = (st, [S.results(s) for s in range(ns)])
         S.close()
     (sf, rf), (sb, rb) = out["1"], out["0"]
-    assert sf["n_trace_launches"] == 1 and sb["n_trace_launches"] > ns
+    assert sf["n_trace_launches"] == (1 if part is None else -(-n[0] * n[1] * n[2] // part))
+    assert sb["n_trace_launches"] > ns
     assert sf["n_steps"] == sb["n_steps"]
     for s in range(ns):
         np.testing.assert_allclose(rf[s]["E"], rb[s]["E"], rtol=1e-12, atol=0)
