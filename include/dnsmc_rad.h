@@ -208,7 +208,8 @@ int dnsmc_set_emitters_list(dnsmc_solver *s, const int *cells, int ncells);
 
 /* Queue the whole radiation solve (all snapshots) and return.
  * The plain estimator with FP64 atomic tallies and records off draws its
- * rays inside the traversal, one launch for all snapshots; the learned
+ * rays inside the traversal, one launch for all snapshots (one per range of
+ * consecutive slots beyond 2^32 - 2^20 rays per snapshot); the learned
  * estimator with FP64 atomic tallies and records off generates the
  * descriptors of a whole snapshot on a second stream while the previous
  * snapshot is traced (one traversal launch per snapshot, two snapshot-sized
@@ -217,6 +218,7 @@ int dnsmc_set_emitters_list(dnsmc_solver *s, const int *cells, int ncells);
  * Environment overrides, read at each run (measurement and diagnosis; the
  * defaults are the measured best):
  *   DNSMC_FUSED=0     batched generation for the plain estimator too
+ *   DNSMC_FUSED_SLOTS=n  at most n slots per fused launch
  *   DNSMC_SNAPWISE=0  batched generation for the learned estimator too
  *   DNSMC_PIPES=1..3  pipeline streams of the batched modes
  *   DNSMC_TABLES=global  keep kappa / Planck tables out of LDS
