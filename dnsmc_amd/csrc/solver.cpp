@@ -629,11 +629,30 @@ static int do_run(dnsmc_solver* s)
     // ray batches: up to 2^30 rays (16 GB of descriptors) per launch; the
     // deterministic tallies keep 8 more bytes per ray, so they use 2^27
     const bool f64 = c.precision == 1;
-    const long long cap = f64 ? (1ll << 26) : (det ? (1ll << 27) : (1ll << 30));
+    long long cap = f64 ? (1ll << 26) : (det ? (1ll << 27) : (1ll << 30));
+    // learned batches: the per-slot CDF rows take at most 8 GB per pipeline;
+    // when 80 % of the device memory the pipelines may use holds it, 32 GB and
+    // up to 2^31 rays per batch instead (about 64 GB per pipeline at 400
+    // rays/cell on the 288 GB MI355X): fewer, longer launches, 3.6 % faster
+    long long cdf_bytes = 8ll << 30;
+    const long long row_bytes = 4ll * mlp_noutp(s->mlp_nbin);
+    if (c.mode == DNSMC_MODE_LEARNED && s->have_mlp && !f64 && !det) {
+        const long long big_slots = std::max<long long>(1, std::min<long long>(std::min<long long>(c.batch_rays, 1ll << 31) / R,
+                                                                                (32ll << 30) / row_bytes));
+        const size_t need = (size_t)dnsmc_solver::kPipes * (size_t)big_slots * (size_t)(16ll * R + row_bytes);
+        size_t have = 0, fr = 0, tot = 0;
+        for (int p = 0; p < dnsmc_solver::kPipes; ++p)
+            have += (size_t)s->pipe[p].d_w0.n * 16 + (size_t)s->pipe[p].d_qcdf.n * 4;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && (double)need <= 0.8 * (double)(fr + have)) {
+            cap = 1ll << 31;
+            cdf_bytes = 32ll << 30;
+        }
+    }
+    if (const char* bc = getenv("DNSMC_BATCH_CAP")) cap = std::max(1ll, std::min(atoll(bc), (1ll << 32) - (1ll << 20)));
+    if (const char* cg = getenv("DNSMC_CDF_GB")) cdf_bytes = std::max(1ll, atoll(cg)) << 30;
     long long slots_per_batch = std::max<long long>(1, std::min<long long>(c.batch_rays, cap) / R);
-    if (c.mode == DNSMC_MODE_LEARNED && s->have_mlp)   // CDF rows: at most 8 GB per pipeline
-        slots_per_batch = std::max<long long>(1, std::min<long long>(slots_per_batch,
-                                                                     (8ll << 30) / (4ll * mlp_noutp(s->mlp_nbin))));
+    if (c.mode == DNSMC_MODE_LEARNED && s->have_mlp)
+        slots_per_batch = std::max<long long>(1, std::min<long long>(slots_per_batch, cdf_bytes / row_bytes));
     // plain estimator with atomic tallies and no records: rays are drawn inside
     // the traversal, one launch for all snapshots (DNSMC_FUSED=0: batched);
     // more than 2^32 - 2^20 rays per snapshot take several launches over

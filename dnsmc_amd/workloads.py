@@ -30,7 +30,7 @@ def channel_config(n, rays_per_cell=100, spectral=0, Tw=573.0, table_dir="", nsn
         bc=((1, 1), (2, 2), (1, 1)), Tw=((0.0, 0.0), (Tw, Tw), (0.0, 0.0)), polar_axis=1,
         nvar=3, photmax=20000, rays_per_cell=rays_per_cell, grid_num=1, maxs=(100000, 100000, 100000, 100000, 100000, 100000),
         toll=1.0e-3, spectral=spectral, kappa_grey=1.0, mode=0, sobol=1, num_streams=16, num_gpu=1, gpu=0,
-        p_row=1, seed=20240601, nsnap=nsnap, precision=0, tally=0, srt=0, batch_rays=1 << 30,
+        p_row=1, seed=20240601, nsnap=nsnap, precision=0, tally=0, srt=0, batch_rays=1 << 31,
         table_dir=table_dir,
     )
     d.update(kw)
