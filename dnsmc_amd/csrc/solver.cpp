@@ -1597,6 +1597,24 @@ int dnsmc_get_array(dnsmc_solver* s, const char* name, int level, float* out, lo
                 for (int j = 0; j < ln[1]; ++j)
                     for (int i = 0; i < ln[0]; ++i)
                         v[o++] = b[(size_t)sn * s->tstride[level] + brick_index(i, j, k, s->nbx[level], s->nby[level])];
+    } else if (nm == "qcdf") {
+        // the learned proposal's cumulative weights of the last batch, one row
+        // of mlp_noutp(nB*nQ) per slot, rows in emitter order (the batch must
+        // cover every slot: one snapshot, batched generation)
+        const dnsmc_solver::Pipe& p = s->pipe[0].d_qcdf.n ? s->pipe[0] : s->pipe[1];
+        const long long nsl = (long long)s->emit.size() * spe_of(s->cfg);
+        const long long row = mlp_noutp(s->mlp_nbin);
+        if (!s->have_mlp || s->emit_dirty || (long long)p.d_qcdf.n < nsl * row)
+            return fail(DNSMC_ESTATE, "no learned batch covering every slot");
+        std::vector<float> b((size_t)(nsl * row));
+        HIPCHK(hipMemcpy(b.data(), p.d_qcdf.p, b.size() * sizeof(float), hipMemcpyDeviceToHost));
+        v.resize(b.size());
+        for (long long ps = 0; ps < nsl; ++ps)
+            memcpy(v.data() + (size_t)s->order[ps] * row, b.data() + (size_t)ps * row, (size_t)row * sizeof(float));
+    } else if (nm == "p_bg" || nm == "pcdf") {
+        const DevBuf<float>& db = nm == "p_bg" ? s->d_pbg : s->d_pcdf;
+        v.resize(db.n);
+        if (db.n) HIPCHK(hipMemcpy(v.data(), db.p, db.n * sizeof(float), hipMemcpyDeviceToHost));
     } else if (nm == "cdf_band") v = s->der.cdf_band;
     else if (nm == "cdf_g") v = s->der.cdf_g;
     else if (nm == "inv_emax") v = s->der.inv_emax;

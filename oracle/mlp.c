@@ -109,3 +109,65 @@ void orc_mlp_loss_grad(int nb, const double *X, const double *W1, const double *
         }
     free(H1); free(H2); free(Z); free(dZ); free(dA2); free(dA1);
 }
+
+/* ---------------------------------------------------- learned draw ---- */
+/* Defensive-mixture CDF search over one head: F(k) = min(fma(U[k], s, eps P[k]), 1)
+ * with s = (1 - eps) / U[n-1] (U: the row's unnormalised inclusive cumulative
+ * weights, P: the plain proposal's inclusive CDF), F(n-1) = 1, F(-1) = 0; the
+ * bin is found with the search semantics of orc_cdf_search on F, and q = F(k) -
+ * F(k-1) is its mixture probability. */
+static float mix_F(const float *U, const float *P, int n, float s, float eps, int k)
+{
+    if (k < 0) return 0.0f;
+    if (k == n - 1) return 1.0f;
+    return fminf(fmaf(U[k], s, eps * P[k]), 1.0f);
+}
+
+static int mix_search(const float *U, const float *P, int n, float eps, float R, float *qk)
+{
+    const float s = (1.0f - eps) / U[n - 1];
+    int lo = 0, w = n;
+    while (w > lo + 1) {
+        const int d = (lo + w) >> 1;
+        const float fd = mix_F(U, P, n, s, eps, d);
+        if (R < fd) w = d;
+        if (R >= fd) lo = d;
+    }
+    int k = lo + 1;
+    if (R < mix_F(U, P, n, s, eps, 0)) k = 0;
+    if (R == 1.0f) k = n - 1;
+    if (k > n - 1) k = n - 1;
+    *qk = mix_F(U, P, n, s, eps, k) - mix_F(U, P, n, s, eps, k - 1);
+    return k;
+}
+
+void orc_learned_draw(int nslots, int R, int ndir, int nbin, int nQ, int noutp, const float *U, const float *pcdf,
+                      const float *p_bg, const int *cells, uint32_t snap, uint64_t seed, float eps, uint32_t *w0,
+                      uint32_t *w1, float *ratio)
+{
+    float *Pd = (float *)malloc(sizeof(float) * (size_t)ndir);
+    for (int e = 0; e < ndir; ++e) Pd[e] = (float)(e + 1) * (1.0f / (float)ndir);
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int sl = 0; sl < nslots; ++sl) {
+        const float *Ud = U + (size_t)sl * noutp;
+        for (int r = 0; r < R; ++r) {
+            const uint32_t ctr[4] = {(uint32_t)r, (uint32_t)cells[sl], snap, 1u};
+            uint32_t x[4];
+            orc_philox4x32_10(ctr, key, x);
+            float qd, qb;
+            const int d = mix_search(Ud, Pd, ndir, eps, orc_u01_open(x[0]), &qd);
+            const int bg = mix_search(Ud + ndir, pcdf, nbin, eps, orc_u01_open(x[1]), &qb);
+            const int band = bg / nQ, g = bg - band * nQ;
+            const float rt = (qb > 0.f && qd > 0.f) ? (p_bg[bg] / qb) * ((1.0f / (float)ndir) / qd) : 0.f;
+            /* direction bin d = 8 cos(theta) bins x 8 phi bins: the bin index in
+             * the top 3 of the 24 uniform bits, the Philox bits below it */
+            const uint32_t mt = ((uint32_t)(d >> 3) << 21) | (x[2] >> 11);
+            const uint32_t mp = ((uint32_t)(d & 7) << 21) | (x[3] >> 11);
+            const size_t q = (size_t)sl * R + r;
+            w0[q] = (mt & 0xFFFFFFu) | ((uint32_t)(band & 0xFF) << 24);
+            w1[q] = (mp & 0xFFFFFFu) | ((uint32_t)((band >> 8) & 0x3) << 24) | ((uint32_t)(g & 0x3F) << 26);
+            ratio[q] = rt;
+        }
+    }
+    free(Pd);
+}

@@ -192,6 +192,15 @@ int orc_build_proposal(int mode, int nB, int nQ, int nT, const float *Tnb, const
 void orc_mlp_forward(int nb, const double *X, const double *W1, const double *b1,
                      const double *W2, const double *b2, const double *W3, const double *b3,
                      int nin, int nh, int nout, double *H1, double *H2, double *Z);
+/* The learned proposal's draw (the generator of the learned estimator; no
+ * reference counterpart): for each of nslots slots (emitting cell cells[sl],
+ * row U[sl*noutp ...] of unnormalised inclusive cumulative weights, ndir
+ * direction bins then nbin (band, g) bins) and each ray r < R: Philox
+ * (r, cell, snap, 1) -> direction bin and (band, g) bin from the defensive
+ * mixtures (1 - eps) q + eps p, descriptor words and the ratio p/q. */
+void orc_learned_draw(int nslots, int R, int ndir, int nbin, int nQ, int noutp, const float *U, const float *pcdf,
+                      const float *p_bg, const int *cells, uint32_t snap, uint64_t seed, float eps, uint32_t *w0,
+                      uint32_t *w1, float *ratio);
 void orc_mlp_loss_grad(int nb, const double *X, const double *W1, const double *b1,
                        const double *W2, const double *b2, const double *W3, const double *b3,
                        int nin, int nh, int nout, int ndir,

@@ -121,6 +121,8 @@ class Oracle:
         L.orc_mlp_forward.argtypes = [C.c_int] + [_f64p] * 7 + [C.c_int] * 3 + [_f64p] * 3
         L.orc_mlp_loss_grad.argtypes = ([C.c_int] + [_f64p] * 7 + [C.c_int] * 4 + [_i32p, _i32p, _f64p]
                                         + [_f64p] * 7)
+        L.orc_learned_draw.argtypes = ([C.c_int] * 6 + [_f32p] * 3 + [_i32p, C.c_uint32, C.c_uint64, C.c_float]
+                                       + [_u32p, _u32p, _f32p])
 
     # ---------------------------------------------------------------- RNG --
     def philox(self, ctr, key):
@@ -333,6 +335,23 @@ class Oracle:
                                    ndir, _ptr(td, _i32p), _ptr(tb, _i32p), _ptr(wt, _f64p), _ptr(loss, _f64p),
                                    *[_ptr(a, _f64p) for a in g])
         return float(loss[0]), g
+
+    def learned_draw(self, U, pcdf, p_bg, cells, R, ndir, nQ, snap, seed, eps):
+        """Descriptors (w0, w1) and p/q of the learned draw for the slots whose
+        cumulative-weight rows U[nslots, noutp] are given (orc_learned_draw)."""
+        U = np.ascontiguousarray(U, np.float32)
+        pcdf = np.ascontiguousarray(pcdf, np.float32)
+        p_bg = np.ascontiguousarray(p_bg, np.float32)
+        cells = np.ascontiguousarray(cells, np.int32)
+        nslots, noutp = U.shape
+        n = nslots * R
+        w0 = np.zeros(n, np.uint32)
+        w1 = np.zeros(n, np.uint32)
+        ratio = np.zeros(n, np.float32)
+        self.lib.orc_learned_draw(nslots, R, ndir, len(pcdf), nQ, noutp, _ptr(U, _f32p), _ptr(pcdf, _f32p),
+                                  _ptr(p_bg, _f32p), _ptr(cells, _i32p), snap, seed, eps, _ptr(w0, _u32p),
+                                  _ptr(w1, _u32p), _ptr(ratio, _f32p))
+        return w0, w1, ratio
 
     def mlp_forward(self, X, W1, b1, W2, b2, W3, b3):
         X, W1, b1, W2, b2, W3, b3 = (np.ascontiguousarray(a, np.float64) for a in (X, W1, b1, W2, b2, W3, b3))

@@ -246,3 +246,32 @@ def test_snapshotwise_learned_matches_batched(trained, monkeypatch, nsnap, pipes
     for s in range(nsnap):
         np.testing.assert_allclose(r1[s]["E"], r0[s]["E"], rtol=1e-12, atol=0)
         np.testing.assert_allclose(r1[s]["A"], r0[s]["A"], rtol=1e-12, atol=0)
+
+
+def test_learned_draw_matches_oracle(oracle, trained, monkeypatch):
+    """raygen_learned ray by ray against the oracle's learned draw
+    (orc_learned_draw, pinned on the CPU by the plain limit and the mixture
+    frequencies) on the same cumulative-weight rows: descriptor words bit for
+    bit and p/q to the last bit (the same FP32 operations)."""
+    net, _, tdir, n = trained
+    monkeypatch.setenv("DNSMC_SNAPWISE", "0")
+    R = 40
+    cfg = _small_case(tdir, 2, R, n=n)
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    S.set_mlp(*net.host_weights(), mix_eps=0.1)
+    S.set_field(0, _field(77, n))
+    S.run()
+    S.wait()
+    ncell = n[0] * n[1] * n[2]
+    rays = S.rays(ncell * R)
+    U = S.array("qcdf").reshape(ncell, -1)
+    p_bg = S.array("p_bg")
+    pcdf = S.array("pcdf")
+    nQ = int(S.array("scalars")[4])
+    S.close()
+    w0, w1, ratio = oracle.learned_draw(U, pcdf, p_bg, np.arange(ncell, dtype=np.int32), R, 64, nQ, 0,
+                                        cfg.seed, np.float32(0.1))
+    np.testing.assert_array_equal(rays["w0"], w0)
+    np.testing.assert_array_equal(rays["w1"], w1)
+    np.testing.assert_array_equal(rays["w"], ratio)

@@ -80,3 +80,76 @@ def test_dropped_term_is_caught(oracle):
     q[2] = q[2].T.copy()
     _, gq = oracle.mlp_loss_grad(X, *q, 5, td, tb, np.ones(6))
     assert np.max(np.abs(gq[0] - g[0])) > 1e-3 * np.max(np.abs(g[0]))
+
+
+# ------------------------------------------------ learned draw (orc_learned_draw)
+def _rows(rng, nslots, ndir, nbin):
+    """Unnormalised inclusive cumulative weights, as the proposal writes them."""
+    noutp = ((ndir + nbin + 63) // 64) * 64
+    U = np.zeros((nslots, noutp), np.float32)
+    for s in range(nslots):
+        U[s, :ndir] = np.cumsum(rng.uniform(0.05, 3.0, ndir)).astype(np.float32)
+        U[s, ndir:ndir + nbin] = np.cumsum(rng.uniform(0.0, 2.0, nbin) ** 3).astype(np.float32)
+    return U
+
+
+def _plain(rng, nbin):
+    p = rng.uniform(0.1, 1.0, nbin)
+    p /= p.sum()
+    pcdf = np.minimum(np.cumsum(p), 1.0).astype(np.float32)
+    pcdf[-1] = 1.0
+    return p.astype(np.float32), pcdf
+
+
+def _decode(w0, w1):
+    band = (w0 >> 24) | (((w1 >> 24) & 3) << 8)
+    g = w1 >> 26
+    return band.astype(np.int64), g.astype(np.int64), (w0 & 0xFFFFFF) >> 21, (w1 & 0xFFFFFF) >> 21
+
+
+def test_learned_draw_pure_plain_mixture_is_the_plain_search(oracle):
+    """eps = 1: the mixture is the plain proposal, so the (band, g) bin is the
+    plain CDF search of the same uniform (orc_cdf_search, pinned by a linear
+    scan), the direction bin the uniform 64-bin search, and p/q = 1 up to the
+    FP32 steps of the CDF."""
+    rng = np.random.default_rng(3)
+    nQ, nB, R = 4, 5, 3000
+    nbin = nB * nQ
+    U = _rows(rng, 2, 64, nbin)
+    p, pcdf = _plain(rng, nbin)
+    cells = np.array([7, 123], np.int32)
+    w0, w1, ratio = oracle.learned_draw(U, pcdf, p, cells, R, 64, nQ, 2, 99, 1.0)
+    band, g, th, ph = _decode(w0, w1)
+    bg = band * nQ + g
+    pd = ((np.arange(64) + 1) / 64.0).astype(np.float32)
+    for sl, cell in enumerate(cells):
+        for r in range(0, R, 97):
+            x = oracle.philox([r, cell, 2, 1], [99, 0])
+            q = sl * R + r
+            assert bg[q] == oracle.cdf_search(pcdf, float(np.float32(oracle.lib.orc_u01_open(int(x[1])))))
+            d = oracle.cdf_search(pd, float(np.float32(oracle.lib.orc_u01_open(int(x[0])))))
+            assert th[q] == d >> 3 and ph[q] == d & 7
+    steps = np.diff(np.concatenate([[0.0], pcdf.astype(np.float64)]))
+    np.testing.assert_allclose(ratio, p[bg] / steps[bg], rtol=1e-5)
+    np.testing.assert_allclose(ratio, 1.0, rtol=1e-5)
+
+
+def test_learned_draw_frequencies_follow_the_mixture(oracle):
+    """Bin frequencies of many draws against the mixture (1 - eps) q + eps p
+    computed independently from the rows (numpy), z < 5 per bin; and
+    E_mixture[p/q] = 1."""
+    rng = np.random.default_rng(5)
+    nQ, nB, R, eps = 3, 4, 400000, 0.3
+    nbin = nB * nQ
+    U = _rows(rng, 1, 64, nbin)
+    p, pcdf = _plain(rng, nbin)
+    w0, w1, ratio = oracle.learned_draw(U, pcdf, p, np.array([11], np.int32), R, 64, nQ, 0, 1234, eps)
+    band, g, th, ph = _decode(w0, w1)
+    for idx, n, Uh, P in ((band * nQ + g, nbin, U[0, 64:64 + nbin], p.astype(np.float64)),
+                          (th * 8 + ph, 64, U[0, :64], np.full(64, 1 / 64.0))):
+        q = np.diff(np.concatenate([[0.0], Uh.astype(np.float64)])) / float(Uh[-1])
+        mix = (1 - eps) * q + eps * P
+        cnt = np.bincount(idx, minlength=n)
+        z = (cnt - R * mix) / np.sqrt(R * mix * (1 - mix))
+        assert np.max(np.abs(z)) < 5.0, z
+    assert abs(float(np.mean(ratio.astype(np.float64))) - 1.0) < 5 * float(np.std(ratio)) / math.sqrt(R)
