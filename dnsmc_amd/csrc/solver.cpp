@@ -1611,6 +1611,22 @@ int dnsmc_get_array(dnsmc_solver* s, const char* name, int level, float* out, lo
         v.resize(b.size());
         for (long long ps = 0; ps < nsl; ++ps)
             memcpy(v.data() + (size_t)s->order[ps] * row, b.data() + (size_t)ps * row, (size_t)row * sizeof(float));
+    } else if (nm == "features") {
+        // the learned proposal's input features of every slot of the last run
+        // (kMlpIn floats per slot), rows in emitter order
+        if (s->der_Tmax <= 0.f || s->emit_dirty) return fail(DNSMC_ESTATE, "run the solver first");
+        const long long nsl = (long long)s->emit.size() * spe_of(s->cfg);
+        DevBuf<float> dX;
+        HIPCHK(dX.ensure((size_t)(nsl * kMlpIn)));
+        const FeatArgs f = feat_args(s, s->der);
+        const int nb = (int)std::min<long long>((nsl + 255) / 256, (long long)s->ncu * 64);
+        hipLaunchKernelGGL(mlp_features, dim3(nb), dim3(256), 0, 0, f, (const int4*)s->d_slotinfo.p, nsl, dX.p);
+        HIPCHK(hipGetLastError());
+        std::vector<float> b((size_t)(nsl * kMlpIn));
+        HIPCHK(hipMemcpy(b.data(), dX.p, b.size() * sizeof(float), hipMemcpyDeviceToHost));
+        v.resize(b.size());
+        for (long long ps = 0; ps < nsl; ++ps)
+            memcpy(v.data() + (size_t)s->order[ps] * kMlpIn, b.data() + (size_t)ps * kMlpIn, kMlpIn * sizeof(float));
     } else if (nm == "p_bg" || nm == "pcdf") {
         const DevBuf<float>& db = nm == "p_bg" ? s->d_pbg : s->d_pcdf;
         v.resize(db.n);

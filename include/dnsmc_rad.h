@@ -238,9 +238,10 @@ int dnsmc_get_stats(dnsmc_solver *s, dnsmc_stats *st);
 
 /* Test and training hooks. */
 /* Derived arrays: name in {"faces0","faces1",..,"centres","T1".."T5","cdf_band","cdf_g",
- * "inv_emax","Ib","kq","Ibw","scalars","p_bg","pcdf","qcdf"}; copies up to n floats,
- * returns count.  "qcdf": the learned proposal's cumulative weights of the last
- * batch, a row per slot in emitter order (the batch must cover every slot). */
+ * "inv_emax","Ib","kq","Ibw","scalars","p_bg","pcdf","qcdf","features"}; copies up to n
+ * floats, returns count.  "qcdf": the learned proposal's cumulative weights of the
+ * last batch, a row per slot in emitter order (the batch must cover every slot);
+ * "features": the proposal's 6 input features per slot of the last run, emitter order. */
 int dnsmc_get_array(dnsmc_solver *s, const char *name, int level, float *out, long long n);
 /* Ray descriptors of the last run's first batch (SoA words) for parity tests.
  * The plain estimator draws its rays inside the traversal unless records are

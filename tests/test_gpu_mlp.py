@@ -275,3 +275,28 @@ def test_learned_draw_matches_oracle(oracle, trained, monkeypatch):
     np.testing.assert_array_equal(rays["w0"], w0)
     np.testing.assert_array_equal(rays["w1"], w1)
     np.testing.assert_array_equal(rays["w"], ratio)
+
+
+def test_proposal_rows_match_the_network(trained, monkeypatch):
+    """mlp_proposal's cumulative weights per cell against the network forward
+    (mlp_forward_rows, pinned by test_forward_matches_oracle) on the same
+    features: each head's row is the running sum of exp(clamp(z, -60, 60))."""
+    net, _, tdir, n = trained
+    monkeypatch.setenv("DNSMC_SNAPWISE", "0")
+    cfg = _small_case(tdir, 2, 8, n=n)
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    S.set_mlp(*net.host_weights(), mix_eps=0.1)
+    S.set_field(0, _field(78, n))
+    S.run()
+    S.wait()
+    ncell = n[0] * n[1] * n[2]
+    U = S.array("qcdf").reshape(ncell, -1).astype(np.float64)
+    X = S.array("features").reshape(ncell, 6)
+    S.close()
+    Z = net.forward(t(X)).numpy().astype(np.float64)
+    nbin = net.nbin
+    e = np.exp(np.clip(Z[:, :64 + nbin], -60.0, 60.0))
+    for lo, hi in ((0, 64), (64, 64 + nbin)):
+        want = np.cumsum(e[:, lo:hi], axis=1)
+        np.testing.assert_allclose(U[:, lo:hi], want, rtol=2e-4, atol=1e-30)
