@@ -12,6 +12,8 @@ tests check exactly that), on the GPU's throughput.
   python -m dnsmc_amd.study heldout      learned proposal on held-out 64x96x64
                                          snapshots: variance ratio, errors
   python -m dnsmc_amd.study convergence  error vs rays per cell (R^-1/2)
+  python -m dnsmc_amd.study legs         one 384x256x256 snapshot: plain, plain
+                                         coarsened, learned (for counter runs)
 
 Errors over the chosen cells: eps_rms = sqrt(sum (Q - Q_ref)^2 - sum var_ref)
 / sqrt(sum Q_ref^2) (the reference's own variance removed), eps_max =
@@ -213,6 +215,28 @@ def study_heldout(rays=(100, 1000), snaps=range(51, 56), **cfg_kw):
     return out
 
 
+def study_legs(workload="ck384x8", rays=100):
+    """One snapshot of the workload solved three ways (plain on the fine grid,
+    plain with the reference's default coarsening, learned on the fine grid):
+    seconds, segments and rays of each solve (the counter profiles of the
+    three traversal kernels are taken around this command)."""
+    from .learned import train_channel_proposal
+    fields = workloads.workload_fields(workload, 1)
+    amc = dict(grid_num=5, maxs=(5, 5, 5, 5, 100000, 100000))
+    W, _ = train_channel_proposal(n_snap=4)
+    out = {}
+    for name, mode, kw, w in (("plain_fine", 0, {}, None), ("plain_coarsened", 0, amc, None),
+                              ("learned_fine", 2, {}, W)):
+        S, _ = make_solver(workload, rays, mode=mode, mlp=w, mix_eps=0.05, fields=fields, **kw)
+        t0 = time.time()
+        S.run()
+        st = S.stats()
+        out[name] = dict(seconds=time.time() - t0, segments=st["n_steps"], rays=st["n_rays"])
+        log(name, out[name])
+        S.close()
+    return out
+
+
 def main(argv):
     what = argv[1] if len(argv) > 1 else "errors"
     if what == "errors":
@@ -221,6 +245,8 @@ def main(argv):
         out = study_precision()
     elif what == "heldout":
         out = study_heldout()
+    elif what == "legs":
+        out = study_legs()
     elif what == "convergence":
         mlp = None
         if "--learned" in argv:
@@ -228,7 +254,7 @@ def main(argv):
             mlp, _ = train_channel_proposal()
         out = study_convergence(mlp=mlp)
     else:
-        raise SystemExit("usage: python -m dnsmc_amd.study errors|precision|heldout|convergence [--learned]")
+        raise SystemExit("usage: python -m dnsmc_amd.study errors|precision|heldout|convergence [--learned]|legs")
     print(json.dumps(out), flush=True)
 
 
