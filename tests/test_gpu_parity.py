@@ -266,6 +266,31 @@ def test_tally_ring_with_few_rays_per_cell(ck_table_dir, R):
         np.testing.assert_allclose(r0["sdev"], r1["sdev"], rtol=1e-9, atol=1e-300)
 
 
+@pytest.mark.parametrize("photmax", [24, 96])
+def test_tally_ring_compat_mode(photmax):
+    """The reference sampling (several tally slots per emitting cell, one per
+    variance block): ring tallies against the deterministic FP64 sums on the
+    particle tables with coarsening, E, A and the block variances."""
+    out = []
+    for tally in (0, 1):
+        cfg = _compat_particle_cfg(n=(16, 12, 12), photmax=photmax, grid_num=3)
+        cfg.tally = tally
+        S = dnsmc_amd.Solver(cfg)
+        S.load_tables()
+        x = S.array("centres0", 0)
+        Tg = np.empty((18, 14, 14), np.float32, order="F")
+        Tg[...] = (500.0 - 2000.0 * x ** 2 + 2000.0 * x)[:, None, None]
+        S.set_field(0, Tg)
+        S.run()
+        out.append((S.results(0), S.stats()))
+        S.close()
+    (r0, s0), (r1, s1) = out
+    assert s0["n_steps"] == s1["n_steps"]
+    np.testing.assert_allclose(r0["E"], r1["E"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(r0["A"], r1["A"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(r0["sdev"], r1["sdev"], rtol=1e-9, atol=1e-300)
+
+
 def test_table_placement_bit_identical(ck_table_dir, monkeypatch):
     """The kappa/Ib slice in LDS and the global-memory tables give the same
     per-ray values, so the deterministic FP64 tallies agree exactly."""
