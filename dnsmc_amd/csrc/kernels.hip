@@ -1460,6 +1460,18 @@ __global__ void field_to_slab(const int* emit, long long nemit, const double* Q,
     vcore[o] = sd[e] / stefan / tc4;
 }
 
+// A temperature field in the reference's layout, T(0:n0+1, 0:n1+1, 0:n2+1) with
+// i fastest (code/param.h:104-105), into the 4x4x4-bricked layout: interior
+// cells only (the ghost layers are not part of the bricked field).
+__global__ void ghost_to_bricks(const float* Tg, int n0, int n1, int n2, float* T, int nbx, int nby)
+{
+    const long long n = (long long)n0 * n1 * n2;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % n0), j = (int)((c / n0) % n1), k = (int)(c / ((long long)n0 * n1));
+        T[brick_index(i, j, k, nbx, nby)] = Tg[(long long)(i + 1) + (long long)(n0 + 2) * ((j + 1) + (long long)(n1 + 2) * (k + 1))];
+    }
+}
+
 // Table-coordinate field of the sliced-table traversal: u = clamp(T/dT - T0/dT,
 // 0, nT-1) - lo, the position of each cell's temperature among the nodes held
 // in LDS.  The traversal's lerp from u is bit-identical to the one from T (the

@@ -56,6 +56,7 @@ __global__ void mlp_samples(FeatArgs f, const int4* slotinfo, long long slot0, i
 __global__ void mlp_features(FeatArgs f, const int4* slotinfo, long long n, float* X);
 __global__ void fill_u32(unsigned* p, long long n, unsigned v);
 __global__ void t_to_u(const float* T, float* U, long long n, float inv_dT, float mT0, float top, float lo);
+__global__ void ghost_to_bricks(const float* Tg, int n0, int n1, int n2, float* T, int nbx, int nby);
 __global__ void slab_to_field(const double* cnew, int n0, int n1, int n2, int i0, int j0, int k0, int ni, int nj,
                               int nk, double Tc, double Tplus, float* T, int nbx, int nby);
 __global__ void field_to_slab(const int* emit, long long nemit, const double* Q, const double* sd, int n0, int n1,
@@ -131,6 +132,7 @@ struct dnsmc_solver {
     std::vector<char> slab_open;   // slabs written since the last commit
     DevBuf<unsigned> d_lohi;
     DevBuf<double> d_slab;         // staging of the DNS coupling slabs
+    DevBuf<float> d_stage;         // staging of host temperature fields
     DevBuf<float> d_T[kMaxLev];
     DevBuf<float> d_U[kMaxLev];    // table coordinates of the sliced-table traversal
     long long ncell[kMaxLev] = {0, 0, 0, 0, 0, 0};     // cells per level
@@ -1340,34 +1342,22 @@ int dnsmc_set_field(dnsmc_solver* s, int snap, const float* T, int is_device)
     }
     const int n0 = s->cfg.n[0], n1 = s->cfg.n[1], n2 = s->cfg.n[2];
     const size_t gsz = (size_t)(n0 + 2) * (n1 + 2) * (n2 + 2);
-    std::vector<float> host;
+    // the field as given (ghost layers included) to the device, then its
+    // interior into the bricked layout by a kernel; the positivity / NaN check
+    // and Tmax come from the commit's range pass (code/mc_gpu.cu:456-480)
     const float* src = T;
-    if (is_device) {
-        host.resize(gsz);
-        HIPCHK(hipMemcpy(host.data(), T, gsz * sizeof(float), hipMemcpyDeviceToHost));
-        src = host.data();
+    if (!is_device) {
+        HIPCHK(s->d_stage.ensure(gsz));
+        HIPCHK(hipMemcpyAsync(s->d_stage.p, T, gsz * sizeof(float), hipMemcpyHostToDevice, s->stream));
+        src = s->d_stage.p;
     }
-    // interior into the 4x4x4-bricked layout (padding cells hold 0 and are
-    // never read), Tmax, NaN check (code/mc_gpu.cu:456-480)
-    std::vector<float> in((size_t)s->tstride[0], 0.f);
-    const int nbx = s->nbx[0], nby = s->nby[0];
-    float tmax = 0.f;
-    for (int k = 1; k <= n2; ++k)
-        for (int j = 1; j <= n1; ++j) {
-            const float* row = src + (size_t)(n0 + 2) * (j + (size_t)(n1 + 2) * k);
-            for (int i = 1; i <= n0; ++i) {
-                float v = row[i];
-                if (!(v > 0.f) || !isfinite(v)) return fail(DNSMC_ERANGE, "temperature field holds a non-positive or non-finite value");
-                tmax = v > tmax ? v : tmax;
-                in[brick_index(i - 1, j - 1, k - 1, nbx, nby)] = v;
-            }
-        }
-    HIPCHK(hipMemcpyAsync(s->d_T[0].p + (size_t)snap * s->tstride[0], in.data(), in.size() * sizeof(float),
-                          hipMemcpyHostToDevice, s->stream));
-    int rc = commit_field(s, snap);
-    if (rc) return rc;
-    s->Tmax_snap[snap] = tmax;
-    return DNSMC_OK;
+    const long long nc = (long long)n0 * n1 * n2;
+    const int nb = (int)std::min<long long>((nc + 255) / 256, (long long)s->ncu * 16);
+    hipLaunchKernelGGL(ghost_to_bricks, dim3(nb), dim3(256), 0, s->stream, src, n0, n1, n2,
+                       s->d_T[0].p + (size_t)snap * s->tstride[0], s->nbx[0], s->nby[0]);
+    HIPCHK(hipGetLastError());
+    s->slab_open[snap] = 0;
+    return commit_field(s, snap);
 }
 
 int dnsmc_fix_temp_slab(dnsmc_solver* s, int snap, const double* cnew, const int* startc, const int* endc, double Tc,

@@ -163,7 +163,9 @@ int dnsmc_set_tables(dnsmc_solver *s, int nB, int nQ, int nT, const float *Tnb, 
                      const float *prob, const float *probg, const float *kavg);
 
 /* Temperature field of snapshot `snap`, Fortran layout with ghosts
- * (n0+2)(n1+2)(n2+2), host or device pointer (is_device). */
+ * (n0+2)(n1+2)(n2+2), host or device pointer (is_device).  A field with a
+ * non-positive or non-finite interior value is rejected (DNSMC_ERANGE) and
+ * leaves the snapshot unset. */
 int dnsmc_set_field(dnsmc_solver *s, int snap, const float *T, int is_device);
 /* DNS coupling without the host all-reduces (code/fix_var.f:49-121, called
  * from code/main_comp.f around mc_gpu_ / get_results_).
