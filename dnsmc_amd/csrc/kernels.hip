@@ -1442,6 +1442,23 @@ __global__ void slab_to_field(const double* cnew, int n0, int n1, int n2, int i0
     }
 }
 
+// get_results_ (code/mc_gpu.cu:759-776): Q and its standard error of the
+// emitters in the j slab [jlo, jlo + nj) (0-based) into real(4) arrays
+// (0:n0+1, 0:nj+1, 0:n2+1), i fastest; other entries keep their values.
+__global__ void results_to_fortran(const int* emit, long long nemit, const double* Q, const double* sd, int n0,
+                                   int n1, int jlo, int nj, int n2, float* resf, float* varf)
+{
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= nemit) return;
+    const int cell = emit[e];
+    const int i = cell % n0, j = (cell / n0) % n1, k = cell / (n0 * n1);
+    const int jl = j - jlo;
+    if (jl < 0 || jl >= nj || k >= n2) return;
+    const long long o = (long long)(i + 1) + (long long)(n0 + 2) * ((jl + 1) + (long long)(nj + 2) * (k + 1));
+    resf[o] = (float)Q[e];
+    varf[o] = (float)sd[e];
+}
+
 // fix_rad (code/fix_var.f:75-121) for one rank's k range, from the device
 // results: divQ(i, jl, k - k0 + 1) = Q / stefan / Tc^4 and vcore likewise from
 // the standard error, for the emitters in the solver's j slab [jlo, jlo + nj)

@@ -57,6 +57,8 @@ __global__ void mlp_features(FeatArgs f, const int4* slotinfo, long long n, floa
 __global__ void fill_u32(unsigned* p, long long n, unsigned v);
 __global__ void t_to_u(const float* T, float* U, long long n, float inv_dT, float mT0, float top, float lo);
 __global__ void ghost_to_bricks(const float* Tg, int n0, int n1, int n2, float* T, int nbx, int nby);
+__global__ void results_to_fortran(const int* emit, long long nemit, const double* Q, const double* sd, int n0,
+                                   int n1, int jlo, int nj, int n2, float* resf, float* varf);
 __global__ void slab_to_field(const double* cnew, int n0, int n1, int n2, int i0, int j0, int k0, int ni, int nj,
                               int nk, double Tc, double Tplus, float* T, int nbx, int nby);
 __global__ void field_to_slab(const int* emit, long long nemit, const double* Q, const double* sd, int n0, int n1,
@@ -1536,20 +1538,28 @@ int dnsmc_get_results(dnsmc_solver* s, int snap, double* Q, double* E, double* A
 int dnsmc_get_results_fortran(dnsmc_solver* s, int snap, float* resfort, float* varfort)
 {
     if (!s || !resfort || !varfort) return fail(DNSMC_EINVAL, "null argument");
-    size_t n = (size_t)s->last_nemit;
-    std::vector<double> Q(n), sd(n);
-    int rc = dnsmc_get_results(s, snap, Q.data(), nullptr, nullptr, sd.data());
+    int rc = do_wait(s);
     if (rc) return rc;
-    const int n0 = s->cfg.n[0], n1 = s->cfg.n[1];
+    if (snap < 0 || snap >= s->cfg.nsnap || s->last_nemit == 0) return fail(DNSMC_ESTATE, "no results for snapshot");
+    const int n0 = s->cfg.n[0], n1 = s->cfg.n[1], n2 = s->cfg.n[2];
     const int nj = n1 / s->cfg.p_row;
-    for (size_t e = 0; e < n; ++e) {
-        int cell = s->emit[e];
-        int i = cell % n0 + 1, j = (cell / n0) % n1 + 1 - (s->ystart - 1), k = cell / (n0 * n1) + 1;
-        if (j < 1 || j > nj) continue;
-        size_t f = (size_t)i + (size_t)(n0 + 2) * ((size_t)j + (size_t)(nj + 2) * k);
-        resfort[f] = (float)Q[e];
-        varfort[f] = (float)sd[e];
-    }
+    const size_t fsz = (size_t)(n0 + 2) * (nj + 2) * (n2 + 2);
+    // the caller's arrays round-trip through the device so that the entries
+    // the results do not cover (ghost layers) keep their values; the interior
+    // is written by a kernel from the FP64 results
+    HIPCHK(s->d_stage.ensure(2 * fsz));
+    float* dq = s->d_stage.p;
+    float* dv = s->d_stage.p + fsz;
+    HIPCHK(hipMemcpyAsync(dq, resfort, fsz * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(dv, varfort, fsz * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    const long long ne = s->last_nemit;
+    hipLaunchKernelGGL(results_to_fortran, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s->stream,
+                       (const int*)s->d_emit.p, ne, (const double*)(s->d_Q.p + (size_t)snap * ne),
+                       (const double*)(s->d_sd.p + (size_t)snap * ne), n0, n1, s->ystart - 1, nj, n2, dq, dv);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(resfort, dq, fsz * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(varfort, dv, fsz * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     return DNSMC_OK;
 }
 
