@@ -166,6 +166,7 @@ struct dnsmc_solver {
     DevBuf<int> d_emit, d_order;
     DevBuf<int4> d_slotinfo;    // processing slot -> {tally slot, cell, bricked index, packed i|j|k}
     bool emit_dirty = true;
+    bool emit_is_slab = false;     // emit holds the j slab from ystart (dnsmc_set_emitters_slab)
     // ray batches are pipelined over kPipes streams, each with its own buffers,
     // so the tail of one traversal launch overlaps the next batch
     struct Pipe {
@@ -1483,11 +1484,13 @@ int dnsmc_set_emitters_slab(dnsmc_solver* s, int ystart)
     const int n0 = s->cfg.n[0], n1 = s->cfg.n[1], n2 = s->cfg.n[2];
     const int nj = n1 / s->cfg.p_row;
     if (ystart < 1 || ystart + nj - 1 > n1) return fail(DNSMC_EINVAL, "ystart out of range");
+    if (s->emit_is_slab && s->ystart == ystart && !s->emit.empty()) return DNSMC_OK;   // unchanged: keep the slot tables
     s->emit.clear();
     for (int k = 0; k < n2; ++k)
         for (int j = ystart - 1; j < ystart - 1 + nj; ++j)
             for (int i = 0; i < n0; ++i) s->emit.push_back(i + n0 * (j + n1 * k));
     s->emit_dirty = true;
+    s->emit_is_slab = true;
     s->ystart = ystart;
     return DNSMC_OK;
 }
@@ -1500,6 +1503,7 @@ int dnsmc_set_emitters_list(dnsmc_solver* s, const int* cells, int ncells)
         if (cells[i] < 0 || cells[i] >= ntot) return fail(DNSMC_EINVAL, "emitter cell out of range");
     s->emit.assign(cells, cells + ncells);
     s->emit_dirty = true;
+    s->emit_is_slab = false;
     s->ystart = 1;
     return DNSMC_OK;
 }
