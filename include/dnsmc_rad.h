@@ -227,7 +227,10 @@ int dnsmc_set_emitters_list(dnsmc_solver *s, const int *cells, int ncells);
  *   DNSMC_REFILL_K=k  idle lanes of a wave before it starts new rays
  *   DNSMC_SUPER=n     rays per claimed super-chunk (multiple of 64)
  *   DNSMC_BLOCK=n     threads per block of the traversal with LDS table
- *                     slices (default 1024: one block and 16 waves per CU) */
+ *                     slices (default 1024: one block and 16 waves per CU)
+ *   DNSMC_BATCH_CAP=n rays per batched launch (at most 2^32 - 2^20)
+ *   DNSMC_CDF_GB=g    GB of learned CDF rows per pipeline (default 8, or 32
+ *                     when the device memory holds it) */
 int dnsmc_run(dnsmc_solver *s);
 /* Wait for the queued solve. */
 int dnsmc_wait(dnsmc_solver *s);
