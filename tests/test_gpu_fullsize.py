@@ -84,3 +84,31 @@ def test_full_channel_eight_snapshots(oracle):
         # segments of the longest rays here, 1e-7 for typical cells
         assert rel[m] <= 1e-4 and np.quantile(rel, 0.9999) <= 2e-6, s
     S2.close()
+
+
+def test_full_channel_coarsened(oracle):
+    """The reference's default coarsening (5 levels, maxs 5/5/5/5/1e5,
+    code/param.h:18-35) at the full shape, two snapshots in the fused launch:
+    sampled cells against the oracle tracing through the same levels."""
+    nsnap = 2
+    fields = workloads.workload_fields("ck384x8", nsnap)
+    S, cfg = study.make_solver("ck384x8", R, nsnap=nsnap, fields=fields, grid_num=5,
+                               maxs=(5, 5, 5, 5, 100000, 100000))
+    n = tuple(cfg.n)
+    S.run()
+    st = S.stats()
+    print("coarsened full channel: %d rays, %d segments, %.1f ms"
+          % (st["n_rays"], st["n_steps"], st["ms_trace"]), flush=True)
+    assert st["n_rays"] == n[0] * n[1] * n[2] * R * nsnap
+    for s in range(nsnap):
+        res = S.results(s)
+        assert np.all(np.isfinite(res["Q"])) and np.all(res["E"] > 0)
+        d = oracle_problem_dict(S, cfg, s)
+        P = oracle.problem(d)
+        for cell in _sample(n, 8, 200 + s):
+            descs = oracle.gen_philox(P, cfg.seed, s, cell, 0, R)
+            E, A, _, _ = oracle.trace_descs(P, cell, descs)
+            assert abs(res["E"][cell] - E) <= 1e-5 * abs(E), (s, cell)
+            assert abs(res["A"][cell] - A) <= 1e-5 * abs(A), (s, cell)
+            assert abs(res["Q"][cell] - (E - A)) <= 1e-5 * abs(E), (s, cell)
+    S.close()
