@@ -227,7 +227,7 @@ def main():
     ap.add_argument("--train-rays", type=int, default=8)
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--cost0", type=float, default=16.0)
-    ap.add_argument("--mix-eps", type=float, default=0.05)
+    ap.add_argument("--mix-eps", type=float, default=0.1)
     ap.add_argument("--ref-cells", type=int, default=1024)
     ap.add_argument("--ref-rays", type=int, default=65536)
     args = ap.parse_args()
