@@ -252,9 +252,11 @@ def main(argv):
         if "--learned" in argv:
             from .learned import train_channel_proposal
             mlp, _ = train_channel_proposal()
-        out = study_convergence(mlp=mlp)
+        eps = float(argv[argv.index("--mix-eps") + 1]) if "--mix-eps" in argv else 0.05
+        out = study_convergence(mlp=mlp, mix_eps=eps)
     else:
-        raise SystemExit("usage: python -m dnsmc_amd.study errors|precision|heldout|convergence [--learned]|legs")
+        raise SystemExit("usage: python -m dnsmc_amd.study errors|precision|heldout|"
+                         "convergence [--learned] [--mix-eps E]|legs")
     print(json.dumps(out), flush=True)
 
 
