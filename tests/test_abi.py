@@ -85,3 +85,15 @@ def test_product_never_uses_the_oracle():
     for f in files:
         text = open(f, errors="replace").read()
         assert "pyoracle" not in text and "liboracle" not in text and "oracle/" not in text, f
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py is a one-GPU measurement: --gpus 2 exits with status 2 and a
+    message before it touches the library or a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
