@@ -184,7 +184,7 @@ def study_convergence(workload="ck384x8", rays=(100, 200, 400, 800), ncells=1024
     return dict(workload=workload, snapshot=0, cells=len(cells), reference_rays_per_cell=rays_ref, legs=res)
 
 
-def study_heldout(rays=(100, 1000), snaps=range(51, 56), **cfg_kw):
+def study_heldout(rays=(100, 1000), snaps=range(51, 56), mix_eps=0.05, **cfg_kw):
     """The learned proposal on held-out channel snapshots (64x96x64, parabolic
     profile + N(0, 10 K), seeds 51-55; trained on seeds 31-50): variance
     ratio vs the plain estimator at equal rays per cell (sums of the per-cell
@@ -194,7 +194,7 @@ def study_heldout(rays=(100, 1000), snaps=range(51, 56), **cfg_kw):
     W, info = train_channel_proposal(**cfg_kw)
     wl = "ck64"
     cells = sample_cells(wl, 1024)
-    out = dict(train=info, snapshots={})
+    out = dict(train=info, mix_eps=mix_eps, snapshots={})
     for seed in snaps:
         fields = [workloads.field_profile_noise(seed, workloads.WORKLOADS[wl]["n"])]
         Qref, vref, _ = reference(wl, cells, 65536, fields=fields, **cfg_kw)
@@ -202,7 +202,7 @@ def study_heldout(rays=(100, 1000), snaps=range(51, 56), **cfg_kw):
         for R in rays:
             res = {}
             for name, mode, Wm in (("plain", 0, None), ("learned", 2, W)):
-                S, _ = make_solver(wl, R, mode=mode, mlp=Wm, mix_eps=0.05, fields=fields, **cfg_kw)
+                S, _ = make_solver(wl, R, mode=mode, mlp=Wm, mix_eps=mix_eps, fields=fields, **cfg_kw)
                 S.run()
                 res[name] = S.results(0)
                 S.close()
@@ -244,7 +244,7 @@ def main(argv):
     elif what == "precision":
         out = study_precision()
     elif what == "heldout":
-        out = study_heldout()
+        out = study_heldout(mix_eps=float(argv[argv.index("--mix-eps") + 1]) if "--mix-eps" in argv else 0.05)
     elif what == "legs":
         out = study_legs()
     elif what == "convergence":
@@ -255,7 +255,7 @@ def main(argv):
         eps = float(argv[argv.index("--mix-eps") + 1]) if "--mix-eps" in argv else 0.05
         out = study_convergence(mlp=mlp, mix_eps=eps)
     else:
-        raise SystemExit("usage: python -m dnsmc_amd.study errors|precision|heldout|"
+        raise SystemExit("usage: python -m dnsmc_amd.study errors|precision|heldout [--mix-eps E]|"
                          "convergence [--learned] [--mix-eps E]|legs")
     print(json.dumps(out), flush=True)
 
