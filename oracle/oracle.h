@@ -181,6 +181,8 @@ void orc_mc_philox_f64(const orc_problem *P, uint64_t seed, uint32_t snap, int R
 
 /* ----------------------------------------------------------- proposal ---- */
 float orc_lerp_f32(const float *row, int nT, float inv_dT, float mT0, float T);
+/* Test hook: build the plain proposal with a planted mistake (proposal.c). */
+void orc_plant_mistake(int k);
 int orc_build_proposal(int mode, int nB, int nQ, int nT, const float *Tnb, const float *kP,
                        const float *wvl, const float *wvr, const float *wvc, const float *kq,
                        const float *prob, const float *probg, const double *wq, float Tmax, int R,

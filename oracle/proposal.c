@@ -12,13 +12,25 @@
  *   plain : CDF of e(b,g) = kappa(Tmax,b,g) Ib(Tmax,b) w_g dnu_b built at
  *           Tmax itself with the kernels' interpolation, Ibmax = 4 pi sum e / R,
  *           so that the estimator is unbiased for the tabulated physics.
- * Pins: the unbiasedness identity sum_bg p(bg) w(bg) = emission at T_i
- * (checked in tests/test_oracle_proposal.py against a direct sum), CDF
- * monotonicity and last entry 1, and the compat rows equal the table rows.
+ * Pins (tests/test_oracle_proposal.py): the emission identity
+ * R sum_bg p(bg) w(bg; T_i) = 4 pi sum_b dnu_b sum_g w_g kappa(T_i,b,g) Ib(T_i,b)
+ * against a direct numpy sum over the parsed table files (unequal band widths,
+ * asymmetric g weights), its grey limit 4 kappa sigma T^4, and the optically
+ * thin closed form through the full estimator (orc_mc_philox); CDF
+ * monotonicity and last entry 1; the compat rows equal the table rows.
+ *
+ * orc_plant_mistake(k) makes the plain proposal wrong on purpose (k = 1: the
+ * width of band b+1 used for band b; 2: the g weights read in reverse; 3: the
+ * g CDF shifted by one entry), so that the tests can show each check fails on
+ * such a mistake; 0 restores the correct build.
  */
 #include "oracle.h"
 #include <math.h>
 #include <stdlib.h>
+
+static int g_plant = 0;
+
+void orc_plant_mistake(int k) { g_plant = k; }
 
 float orc_lerp_f32(const float *row, int nT, float inv_dT, float mT0, float T)
 {
@@ -54,12 +66,14 @@ int orc_build_proposal(int mode, int nB, int nQ, int nT, const float *Tnb, const
     double *e = (double *)malloc(sizeof(double) * nB * nQ);
     for (int b = 0; b < nB; ++b) {
         float ibm = orc_lerp_f32(Ib + b * nT, nT, inv_dT, mT0, Tmax);
-        double dnu = ((double)wvr[b] - (double)wvl[b]) * 100.0;
+        const int bw = g_plant == 1 ? (b + 1) % nB : b;
+        double dnu = ((double)wvr[bw] - (double)wvl[bw]) * 100.0;
         for (int g = 0; g < nQ; ++g) {
             float km = orc_lerp_f32(kqrow + (b * nQ + g) * nT, nT, inv_dT, mT0, Tmax);
             double em = (double)km * (double)ibm;
             inv_emax[b * nQ + g] = em > 0.0 ? (float)(1.0 / em) : 0.0f;
-            e[b * nQ + g] = 4.0 * 3.14159265358979323846 * em * wq[g] * dnu;
+            const double wg = g_plant == 2 ? wq[nQ - 1 - g] : wq[g];
+            e[b * nQ + g] = 4.0 * 3.14159265358979323846 * em * wg * dnu;
             etot += e[b * nQ + g];
         }
     }
@@ -79,8 +93,9 @@ int orc_build_proposal(int mode, int nB, int nQ, int nT, const float *Tnb, const
             for (int g = 0; g < nQ; ++g) eb += e[b * nQ + g];
             double cg = 0.0;
             for (int g = 0; g < nQ; ++g) {
-                cg += e[b * nQ + g];
+                if (g_plant != 3) cg += e[b * nQ + g];
                 cdf_g[b * nQ + g] = eb > 0.0 ? (float)(cg / eb) : (float)(g + 1) / nQ;
+                if (g_plant == 3) cg += e[b * nQ + g];
             }
             cdf_g[b * nQ + nQ - 1] = 1.0f;
             cum += eb;

@@ -91,6 +91,8 @@ class Oracle:
         L.orc_grid_ref_quad.argtypes = [C.c_int, C.c_int, C.c_float, _f32p, _f32p]
         L.orc_grid_uniform.argtypes = [C.c_int, C.c_float, _f32p, _f32p]
         L.orc_grid_tanh.argtypes = [C.c_int, C.c_float, C.c_double, _f32p, _f32p]
+        L.orc_plant_mistake.restype = None
+        L.orc_plant_mistake.argtypes = [C.c_int]
         L.orc_planck_band.restype = C.c_float
         L.orc_planck_band.argtypes = [C.c_float, C.c_float]
         L.orc_interp3d.argtypes = [_i32p, _f32p, _f32p, _f32p, _f32p, _i32p, _f32p, _f32p, _f32p, _f32p]
@@ -207,6 +209,11 @@ class Oracle:
             self.lib.orc_slab_exchange_cut(len(k), _ptr(f, _f64p), _ptr(k, _f64p), _ptr(b, _f64p),
                                            Ib_lo, Ib_hi, tau_cut, npanel, _ptr(F, _f64p))
         return F
+
+    def plant_mistake(self, k):
+        """Build the plain proposal with mistake k (1: width of the next band,
+        2: g weights reversed, 3: g CDF shifted by one; 0: correct)."""
+        self.lib.orc_plant_mistake(int(k))
 
     def build_proposal(self, mode, t, Tmax, R, wq):
         """t: dict Tnb, kP, wvl, wvr, wvc, kq[nB][nT][nQ], prob[nT][nB], probg[nT][nB][nQ]."""
