@@ -19,7 +19,7 @@ TALLY_F64_ATOMIC, TALLY_F64, TALLY_KAHAN_F32, TALLY_F32 = 0, 1, 2, 3
 EXPORTED = [
     "mc_gpu_", "get_results_", "mc_gpu", "get_results",
     "dnsmc_default_config", "dnsmc_parse_config", "dnsmc_create", "dnsmc_destroy", "dnsmc_last_error",
-    "dnsmc_load_tables", "dnsmc_set_tables", "dnsmc_set_field", "dnsmc_set_mlp",
+    "dnsmc_load_tables", "dnsmc_probe_tables", "dnsmc_set_tables", "dnsmc_set_field", "dnsmc_set_mlp",
     "dnsmc_set_emitters_slab", "dnsmc_set_emitters_list", "dnsmc_run", "dnsmc_wait",
     "dnsmc_get_results", "dnsmc_get_results_fortran", "dnsmc_get_stats", "dnsmc_get_array",
     "dnsmc_get_rays", "dnsmc_get_ray_records", "dnsmc_set_record", "dnsmc_sync", "dnsmc_bench_copy",
@@ -96,6 +96,7 @@ def lib():
     L.dnsmc_destroy.restype = None
     L.dnsmc_last_error.restype = C.c_char_p
     L.dnsmc_load_tables.argtypes = [vp]
+    L.dnsmc_probe_tables.argtypes = [C.POINTER(Config), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.dnsmc_set_tables.argtypes = [vp, i32, i32, i32] + [f32p] * 9
     L.dnsmc_set_field.argtypes = [vp, i32, vp, i32]
     L.dnsmc_set_mlp.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_float]

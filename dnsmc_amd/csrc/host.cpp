@@ -375,12 +375,28 @@ int read_tables(const dnsmc_config& c, Tables& t, std::string* err)
         for (int j = 0; j < nB; ++j) {
             if (!read_floats_line(fp, v) || v.size() < (size_t)(3 + (gas ? nQ : 0))) {
                 fclose(fp);
-                *err = "malformed " + pf;
+                *err = "malformed or short " + pf;
+                return DNSMC_EIO;
+            }
+            // each row names its temperature and band: a band file missing
+            // from the NarrBand sequence shows here as a row mismatch
+            if ((int)v[1] != j || fabs(v[0] - (double)t.Tnb[i]) > 1e-3 * (1.0 + fabs(v[0]))) {
+                fclose(fp);
+                *err = "row of " + pf + " does not match (T = " + std::to_string(t.Tnb[i]) + ", band " +
+                       std::to_string(j) + "): NarrBand files and probability file disagree";
                 return DNSMC_EIO;
             }
             t.prob[(size_t)i * nB + j] = (float)v[2];
             for (int g = 0; g < nQ; ++g)
                 t.probg[((size_t)i * nB + j) * nQ + g] = gas ? (float)v[3 + g] : (float)g;   // :234 probg_h = g
+        }
+    // no rows beyond nT x nB (more bands in the probability file than band files)
+    while (read_floats_line(fp, v))
+        if (!v.empty()) {
+            fclose(fp);
+            *err = pf + " holds more rows than " + std::to_string(nT) + " temperatures x " + std::to_string(nB) +
+                   " bands";
+            return DNSMC_EIO;
         }
     fclose(fp);
     return DNSMC_OK;

@@ -208,6 +208,7 @@ struct dnsmc_solver {
     int last_R = 0, last_spe = 1;
     long long last_nemit = 0;
     int ystart = 1;
+    int last_ystart = 1;   // emitter j slab of the last run (its results' placement)
 };
 
 // ------------------------------------------------------------------ util ----
@@ -610,6 +611,10 @@ static int do_run(dnsmc_solver* s)
     for (int i = 0; i < c.nsnap; ++i)
         if (!s->field_set[i]) return fail(DNSMC_ESTATE, "temperature field of snapshot " + std::to_string(i) + " not set");
     if (s->emit.empty()) return fail(DNSMC_ESTATE, "no emitting cells");
+    // tally slots, processing-slot tags and slot-table entries are 32-bit
+    // indices over every snapshot of the run
+    if ((long long)s->emit.size() * spe_of(c) * (long long)c.nsnap > 0x7FFFFFFFll)
+        return fail(DNSMC_EINVAL, "emitters x blocks x snapshots must stay below 2^31 (split the run)");
     float Tmax = 0.f;
     for (int i = 0; i < c.nsnap; ++i) Tmax = std::max(Tmax, s->Tmax_snap[i]);
     for (int a = 0; a < 3; ++a)
@@ -1109,6 +1114,7 @@ static int do_run(dnsmc_solver* s)
     s->last_R = R;
     s->last_spe = spe;
     s->last_nemit = nemit;
+    s->last_ystart = s->ystart;
     s->running = true;
     return DNSMC_OK;
 }
@@ -1291,6 +1297,19 @@ void dnsmc_destroy(dnsmc_solver* s)
     delete s;
 }
 
+int dnsmc_probe_tables(const dnsmc_config* cfg, int* nB, int* nQ, int* nT)
+{
+    if (!cfg) return fail(DNSMC_EINVAL, "null argument");
+    Tables t;
+    std::string err;
+    int rc = read_tables(*cfg, t, &err);
+    if (rc) return fail(rc, err);
+    if (nB) *nB = t.synth_grey ? 1000 : t.nB;
+    if (nQ) *nQ = t.synth_grey ? 1 : t.nQ;
+    if (nT) *nT = t.synth_grey ? 55 : t.nT;
+    return DNSMC_OK;
+}
+
 int dnsmc_load_tables(dnsmc_solver* s)
 {
     if (!s) return fail(DNSMC_EINVAL, "null solver");
@@ -1438,7 +1457,7 @@ int dnsmc_fix_rad_slab(dnsmc_solver* s, int snap, const int* startc, const int* 
     const double tc4 = Tc * Tc * Tc * Tc;
     hipLaunchKernelGGL(field_to_slab, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s->stream,
                        (const int*)s->d_emit.p, ne, (const double*)(s->d_Q.p + (size_t)snap * ne),
-                       (const double*)(s->d_sd.p + (size_t)snap * ne), n[0], n[1], s->ystart - 1, nj, startc[2] - 1, nk,
+                       (const double*)(s->d_sd.p + (size_t)snap * ne), n[0], n[1], s->last_ystart - 1, nj, startc[2] - 1, nk,
                        stefan, tc4, dq, dv);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(divQ, dq, cnt * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -1559,7 +1578,7 @@ int dnsmc_get_results_fortran(dnsmc_solver* s, int snap, float* resfort, float* 
     const long long ne = s->last_nemit;
     hipLaunchKernelGGL(results_to_fortran, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s->stream,
                        (const int*)s->d_emit.p, ne, (const double*)(s->d_Q.p + (size_t)snap * ne),
-                       (const double*)(s->d_sd.p + (size_t)snap * ne), n0, n1, s->ystart - 1, nj, n2, dq, dv);
+                       (const double*)(s->d_sd.p + (size_t)snap * ne), n0, n1, s->last_ystart - 1, nj, n2, dq, dv);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(resfort, dq, fsz * sizeof(float), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipMemcpyAsync(varfort, dv, fsz * sizeof(float), hipMemcpyDeviceToHost, s->stream));

@@ -47,6 +47,15 @@ def config_from_dict(d, base=None):
     return c
 
 
+def probe_tables(cfg):
+    """Read the spectral tables cfg names on the host only (no device):
+    returns (nB, nQ, nT) or raises DnsmcError (DNSMC_EIO for a missing, short
+    or inconsistent file)."""
+    nB, nQ, nT = C.c_int(0), C.c_int(0), C.c_int(0)
+    check(lib().dnsmc_probe_tables(C.byref(cfg), C.byref(nB), C.byref(nQ), C.byref(nT)))
+    return nB.value, nQ.value, nT.value
+
+
 def _f32(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 

@@ -158,6 +158,13 @@ const char *dnsmc_last_error(void);
  * its last temperature row in NarrBand%d.txt (the sorted mode's band key,
  * code/read_tables.cpp:50; NULL: the mean over g of kq at the last row). */
 int dnsmc_load_tables(dnsmc_solver *s);
+/* Reads the tables cfg names (as dnsmc_load_tables does) on the host only, no
+ * device and no solver needed, and reports their sizes (nB, nQ, nT; any
+ * pointer may be NULL).  The reference's readT (code/read_tables.cpp:20-92)
+ * reads with unchecked fopen/fscanf; here a missing file, a short or
+ * malformed one, or a probability file whose (T, band) rows do not match the
+ * band files returns DNSMC_EIO with the file named in dnsmc_last_error(). */
+int dnsmc_probe_tables(const dnsmc_config *cfg, int *nB, int *nQ, int *nT);
 int dnsmc_set_tables(dnsmc_solver *s, int nB, int nQ, int nT, const float *Tnb, const float *kP,
                      const float *wvl, const float *wvr, const float *wvc, const float *kq,
                      const float *prob, const float *probg, const float *kavg);

@@ -9,11 +9,38 @@
  * these are checked against central finite differences of orc_mlp_loss_grad's
  * loss (tests/test_oracle_mlp.py), so a dropped term or transposed operand
  * fails.  No reference counterpart (the learned proposal is new).
+ *
+ * The _bf16 variants round the matrix-product operands to BF16 where the
+ * kernels do (FP64 -> FP32 -> BF16, round to nearest even at each step):
+ * H1 and H2 as operands of the next layer's product and of the weight
+ * gradients, dZ and dA2 as operands of the backward products; the tanh
+ * derivatives, the bias gradients and dW1 = X^T dA1 use the unrounded values.
+ * The rounding itself is pinned against numpy (tests/test_oracle_mlp.py).
  */
 #include "oracle.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <stdint.h>
+
+/* BF16 value of x as the kernels form it: x -> FP32 (nearest even), then the
+ * FP32 bits rounded to the top 16 (nearest even; non-finite values kept). */
+double orc_bf16_round(double x)
+{
+    float f = (float)x;
+    uint32_t u;
+    memcpy(&u, &f, sizeof u);
+    if ((u & 0x7F800000u) != 0x7F800000u) u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    memcpy(&f, &u, sizeof f);
+    return (double)f;
+}
+
+static void round_all(double *a, size_t n, int q)
+{
+    if (!q) return;
+    for (size_t i = 0; i < n; ++i) a[i] = orc_bf16_round(a[i]);
+}
 
 static void dense(int nb, int nin, int nout, const double *X, const double *W, const double *b,
                   double *Y, int act)
@@ -26,13 +53,44 @@ static void dense(int nb, int nin, int nout, const double *X, const double *W, c
         }
 }
 
+/* Forward; with q the products read BF16-rounded H1 / H2 (H1q, H2q), while
+ * H1, H2 keep the unrounded activations. */
+static void forward_q(int nb, const double *X, const double *W1, const double *b1, const double *W2,
+                      const double *b2, const double *W3, const double *b3, int nin, int nh, int nout,
+                      double *H1, double *H2, double *Z, double *H1q, double *H2q, int q)
+{
+    dense(nb, nin, nh, X, W1, b1, H1, 1);
+    memcpy(H1q, H1, sizeof(double) * (size_t)nb * nh);
+    round_all(H1q, (size_t)nb * nh, q);
+    dense(nb, nh, nh, H1q, W2, b2, H2, 1);
+    memcpy(H2q, H2, sizeof(double) * (size_t)nb * nh);
+    round_all(H2q, (size_t)nb * nh, q);
+    dense(nb, nh, nout, H2q, W3, b3, Z, 0);
+}
+
+static void forward_api(int nb, const double *X, const double *W1, const double *b1, const double *W2,
+                        const double *b2, const double *W3, const double *b3, int nin, int nh, int nout,
+                        double *H1, double *H2, double *Z, int q)
+{
+    double *H1q = (double *)malloc(sizeof(double) * (size_t)nb * nh);
+    double *H2q = (double *)malloc(sizeof(double) * (size_t)nb * nh);
+    forward_q(nb, X, W1, b1, W2, b2, W3, b3, nin, nh, nout, H1, H2, Z, H1q, H2q, q);
+    free(H1q);
+    free(H2q);
+}
+
 void orc_mlp_forward(int nb, const double *X, const double *W1, const double *b1, const double *W2,
                      const double *b2, const double *W3, const double *b3, int nin, int nh, int nout,
                      double *H1, double *H2, double *Z)
 {
-    dense(nb, nin, nh, X, W1, b1, H1, 1);
-    dense(nb, nh, nh, H1, W2, b2, H2, 1);
-    dense(nb, nh, nout, H2, W3, b3, Z, 0);
+    forward_api(nb, X, W1, b1, W2, b2, W3, b3, nin, nh, nout, H1, H2, Z, 0);
+}
+
+void orc_mlp_forward_bf16(int nb, const double *X, const double *W1, const double *b1, const double *W2,
+                          const double *b2, const double *W3, const double *b3, int nin, int nh, int nout,
+                          double *H1, double *H2, double *Z)
+{
+    forward_api(nb, X, W1, b1, W2, b2, W3, b3, nin, nh, nout, H1, H2, Z, 1);
 }
 
 static double softmax_ce(const double *z, int n, int t, double scale, double *dz)
@@ -46,18 +104,22 @@ static double softmax_ce(const double *z, int n, int t, double scale, double *dz
     return lse - z[t];
 }
 
-void orc_mlp_loss_grad(int nb, const double *X, const double *W1, const double *b1, const double *W2,
-                       const double *b2, const double *W3, const double *b3, int nin, int nh, int nout,
-                       int ndir, const int *td, const int *tb, const double *wt, double *loss,
-                       double *gW1, double *gb1, double *gW2, double *gb2, double *gW3, double *gb3)
+static void loss_grad_q(int nb, const double *X, const double *W1, const double *b1, const double *W2,
+                        const double *b2, const double *W3, const double *b3, int nin, int nh, int nout,
+                        int ndir, const int *td, const int *tb, const double *wt, double *loss,
+                        double *gW1, double *gb1, double *gW2, double *gb2, double *gW3, double *gb3, int q)
 {
     double *H1 = (double *)malloc(sizeof(double) * (size_t)nb * nh);
     double *H2 = (double *)malloc(sizeof(double) * (size_t)nb * nh);
+    double *H1q = (double *)malloc(sizeof(double) * (size_t)nb * nh);
+    double *H2q = (double *)malloc(sizeof(double) * (size_t)nb * nh);
     double *Z = (double *)malloc(sizeof(double) * (size_t)nb * nout);
     double *dZ = (double *)malloc(sizeof(double) * (size_t)nb * nout);
+    double *dZq = (double *)malloc(sizeof(double) * (size_t)nb * nout);
     double *dA2 = (double *)malloc(sizeof(double) * (size_t)nb * nh);
+    double *dA2q = (double *)malloc(sizeof(double) * (size_t)nb * nh);
     double *dA1 = (double *)malloc(sizeof(double) * (size_t)nb * nh);
-    orc_mlp_forward(nb, X, W1, b1, W2, b2, W3, b3, nin, nh, nout, H1, H2, Z);
+    forward_q(nb, X, W1, b1, W2, b2, W3, b3, nin, nh, nout, H1, H2, Z, H1q, H2q, q);
     double wsum = 0.0;
     for (int s = 0; s < nb; ++s) wsum += wt[s];
     double L = 0.0;
@@ -69,33 +131,37 @@ void orc_mlp_loss_grad(int nb, const double *X, const double *W1, const double *
         L += sc * softmax_ce(z + ndir, nout - ndir, tb[s], sc, dz + ndir);
     }
     *loss = L;
+    memcpy(dZq, dZ, sizeof(double) * (size_t)nb * nout);
+    round_all(dZq, (size_t)nb * nout, q);
     memset(gW3, 0, sizeof(double) * (size_t)nh * nout);
     memset(gb3, 0, sizeof(double) * nout);
     for (int s = 0; s < nb; ++s)
         for (int o = 0; o < nout; ++o) {
-            double d = dZ[(size_t)s * nout + o];
-            gb3[o] += d;
-            for (int i = 0; i < nh; ++i) gW3[(size_t)i * nout + o] += H2[(size_t)s * nh + i] * d;
+            gb3[o] += dZ[(size_t)s * nout + o];
+            double d = dZq[(size_t)s * nout + o];
+            for (int i = 0; i < nh; ++i) gW3[(size_t)i * nout + o] += H2q[(size_t)s * nh + i] * d;
         }
     for (int s = 0; s < nb; ++s)
         for (int i = 0; i < nh; ++i) {
             double acc = 0.0;
-            for (int o = 0; o < nout; ++o) acc += dZ[(size_t)s * nout + o] * W3[(size_t)i * nout + o];
+            for (int o = 0; o < nout; ++o) acc += dZq[(size_t)s * nout + o] * W3[(size_t)i * nout + o];
             double h = H2[(size_t)s * nh + i];
             dA2[(size_t)s * nh + i] = acc * (1.0 - h * h);
         }
+    memcpy(dA2q, dA2, sizeof(double) * (size_t)nb * nh);
+    round_all(dA2q, (size_t)nb * nh, q);
     memset(gW2, 0, sizeof(double) * (size_t)nh * nh);
     memset(gb2, 0, sizeof(double) * nh);
     for (int s = 0; s < nb; ++s)
         for (int o = 0; o < nh; ++o) {
-            double d = dA2[(size_t)s * nh + o];
-            gb2[o] += d;
-            for (int i = 0; i < nh; ++i) gW2[(size_t)i * nh + o] += H1[(size_t)s * nh + i] * d;
+            gb2[o] += dA2[(size_t)s * nh + o];
+            double d = dA2q[(size_t)s * nh + o];
+            for (int i = 0; i < nh; ++i) gW2[(size_t)i * nh + o] += H1q[(size_t)s * nh + i] * d;
         }
     for (int s = 0; s < nb; ++s)
         for (int i = 0; i < nh; ++i) {
             double acc = 0.0;
-            for (int o = 0; o < nh; ++o) acc += dA2[(size_t)s * nh + o] * W2[(size_t)i * nh + o];
+            for (int o = 0; o < nh; ++o) acc += dA2q[(size_t)s * nh + o] * W2[(size_t)i * nh + o];
             double h = H1[(size_t)s * nh + i];
             dA1[(size_t)s * nh + i] = acc * (1.0 - h * h);
         }
@@ -107,7 +173,25 @@ void orc_mlp_loss_grad(int nb, const double *X, const double *W1, const double *
             gb1[o] += d;
             for (int i = 0; i < nin; ++i) gW1[(size_t)i * nh + o] += X[(size_t)s * nin + i] * d;
         }
-    free(H1); free(H2); free(Z); free(dZ); free(dA2); free(dA1);
+    free(H1); free(H2); free(H1q); free(H2q); free(Z); free(dZ); free(dZq); free(dA2); free(dA2q); free(dA1);
+}
+
+void orc_mlp_loss_grad(int nb, const double *X, const double *W1, const double *b1, const double *W2,
+                       const double *b2, const double *W3, const double *b3, int nin, int nh, int nout,
+                       int ndir, const int *td, const int *tb, const double *wt, double *loss,
+                       double *gW1, double *gb1, double *gW2, double *gb2, double *gW3, double *gb3)
+{
+    loss_grad_q(nb, X, W1, b1, W2, b2, W3, b3, nin, nh, nout, ndir, td, tb, wt, loss, gW1, gb1, gW2, gb2, gW3,
+                gb3, 0);
+}
+
+void orc_mlp_loss_grad_bf16(int nb, const double *X, const double *W1, const double *b1, const double *W2,
+                            const double *b2, const double *W3, const double *b3, int nin, int nh, int nout,
+                            int ndir, const int *td, const int *tb, const double *wt, double *loss,
+                            double *gW1, double *gb1, double *gW2, double *gb2, double *gW3, double *gb3)
+{
+    loss_grad_q(nb, X, W1, b1, W2, b2, W3, b3, nin, nh, nout, ndir, td, tb, wt, loss, gW1, gb1, gW2, gb2, gW3,
+                gb3, 1);
 }
 
 /* ---------------------------------------------------- learned draw ---- */

@@ -209,6 +209,18 @@ void orc_mlp_loss_grad(int nb, const double *X, const double *W1, const double *
                        const int *tgt_dir, const int *tgt_bin, const double *wt,
                        double *loss, double *gW1, double *gb1, double *gW2, double *gb2,
                        double *gW3, double *gb3);
+/* The same two with the kernels' BF16 rounding of the product operands
+ * (H1, H2 forward; dZ, dA2 backward; see mlp.c), and the rounding itself. */
+void orc_mlp_forward_bf16(int nb, const double *X, const double *W1, const double *b1,
+                          const double *W2, const double *b2, const double *W3, const double *b3,
+                          int nin, int nh, int nout, double *H1, double *H2, double *Z);
+void orc_mlp_loss_grad_bf16(int nb, const double *X, const double *W1, const double *b1,
+                            const double *W2, const double *b2, const double *W3, const double *b3,
+                            int nin, int nh, int nout, int ndir,
+                            const int *tgt_dir, const int *tgt_bin, const double *wt,
+                            double *loss, double *gW1, double *gb1, double *gW2, double *gb2,
+                            double *gW3, double *gb3);
+double orc_bf16_round(double x);
 
 #ifdef __cplusplus
 }

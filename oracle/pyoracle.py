@@ -121,8 +121,12 @@ class Oracle:
         L.orc_build_proposal.argtypes = ([C.c_int] * 4 + [_f32p] * 8 + [_f64p, C.c_float, C.c_int]
                                          + [_f32p] * 5 + [_f64p, _f32p])
         L.orc_mlp_forward.argtypes = [C.c_int] + [_f64p] * 7 + [C.c_int] * 3 + [_f64p] * 3
+        L.orc_mlp_forward_bf16.argtypes = L.orc_mlp_forward.argtypes
+        L.orc_bf16_round.argtypes = [C.c_double]
+        L.orc_bf16_round.restype = C.c_double
         L.orc_mlp_loss_grad.argtypes = ([C.c_int] + [_f64p] * 7 + [C.c_int] * 4 + [_i32p, _i32p, _f64p]
                                         + [_f64p] * 7)
+        L.orc_mlp_loss_grad_bf16.argtypes = L.orc_mlp_loss_grad.argtypes
         L.orc_learned_draw.argtypes = ([C.c_int] * 6 + [_f32p] * 3 + [_i32p, C.c_uint32, C.c_uint64, C.c_float]
                                        + [_u32p, _u32p, _f32p])
 
@@ -329,7 +333,12 @@ class Oracle:
         return E, A, Q2, ns.value
 
     # -------------------------------------------------------------- MLP ----
-    def mlp_loss_grad(self, X, W1, b1, W2, b2, W3, b3, ndir, td, tb, wt):
+    def bf16_round(self, x):
+        return self.lib.orc_bf16_round(float(x))
+
+    def mlp_loss_grad(self, X, W1, b1, W2, b2, W3, b3, ndir, td, tb, wt, bf16=False):
+        """Loss and gradients; bf16=True rounds the product operands as the
+        kernels do (orc_mlp_loss_grad_bf16)."""
         X, W1, b1, W2, b2, W3, b3, wt = (np.ascontiguousarray(a, np.float64) for a in (X, W1, b1, W2, b2, W3, b3, wt))
         td = np.ascontiguousarray(td, np.int32)
         tb = np.ascontiguousarray(tb, np.int32)
@@ -338,9 +347,10 @@ class Oracle:
         nout = W3.shape[1]
         g = [np.zeros_like(a) for a in (W1, b1, W2, b2, W3, b3)]
         loss = np.zeros(1)
-        self.lib.orc_mlp_loss_grad(nb, *[_ptr(a, _f64p) for a in (X, W1, b1, W2, b2, W3, b3)], nin, nh, nout,
-                                   ndir, _ptr(td, _i32p), _ptr(tb, _i32p), _ptr(wt, _f64p), _ptr(loss, _f64p),
-                                   *[_ptr(a, _f64p) for a in g])
+        fn = self.lib.orc_mlp_loss_grad_bf16 if bf16 else self.lib.orc_mlp_loss_grad
+        fn(nb, *[_ptr(a, _f64p) for a in (X, W1, b1, W2, b2, W3, b3)], nin, nh, nout,
+           ndir, _ptr(td, _i32p), _ptr(tb, _i32p), _ptr(wt, _f64p), _ptr(loss, _f64p),
+           *[_ptr(a, _f64p) for a in g])
         return float(loss[0]), g
 
     def learned_draw(self, U, pcdf, p_bg, cells, R, ndir, nQ, snap, seed, eps):
@@ -360,14 +370,15 @@ class Oracle:
                                   _ptr(w1, _u32p), _ptr(ratio, _f32p))
         return w0, w1, ratio
 
-    def mlp_forward(self, X, W1, b1, W2, b2, W3, b3):
+    def mlp_forward(self, X, W1, b1, W2, b2, W3, b3, bf16=False):
         X, W1, b1, W2, b2, W3, b3 = (np.ascontiguousarray(a, np.float64) for a in (X, W1, b1, W2, b2, W3, b3))
         nb, nin = X.shape
         nh = W1.shape[1]
         nout = W3.shape[1]
         H1 = np.zeros((nb, nh)); H2 = np.zeros((nb, nh)); Z = np.zeros((nb, nout))
-        self.lib.orc_mlp_forward(nb, *[_ptr(a, _f64p) for a in (X, W1, b1, W2, b2, W3, b3)], nin, nh, nout,
-                                 _ptr(H1, _f64p), _ptr(H2, _f64p), _ptr(Z, _f64p))
+        fn = self.lib.orc_mlp_forward_bf16 if bf16 else self.lib.orc_mlp_forward
+        fn(nb, *[_ptr(a, _f64p) for a in (X, W1, b1, W2, b2, W3, b3)], nin, nh, nout,
+           _ptr(H1, _f64p), _ptr(H2, _f64p), _ptr(Z, _f64p))
         return H1, H2, Z
 
 

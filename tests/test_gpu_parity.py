@@ -200,6 +200,70 @@ def test_grey_slab_closed_form(oracle):
     assert abs(np.mean(z)) < 0.5
 
 
+def test_grey_ref_tables_parity(oracle):
+    """grey==1 (code/read_tables.cpp:104-183): 1000 synthesised bands of 15 1/cm,
+    kappa = 1, temperature nodes spanning [450 K, 1.1 Tmax).  Rays, records and
+    tallies against the oracle built from the library's synthesised arrays."""
+    cfg = small_channel(R=32, spectral=1)
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    T = workloads.field_grey32(4, tuple(cfg.n))
+    S.set_field(0, workloads.ghosted(T))
+    S.set_record(True)
+    S.run()
+    S.wait()
+    sc = S.array("scalars")
+    nB, nQ, nT = int(sc[3]), int(sc[4]), int(sc[5])
+    assert (nB, nQ, nT) == (1000, 1, 55)
+    Tmax = float(sc[0])
+    # the synthesised node grid and Planck table (code/read_tables.cpp:107-125)
+    Tnb = np.float32(450.0) + (np.float32(1.1) * np.float32(Tmax) - np.float32(450.0)) * np.arange(
+        nT, dtype=np.float32) / np.float32(nT)
+    nu = 15.0 * np.arange(nB) + 7.5
+    ib = S.array("Ib").reshape(nB, nT).astype(np.float64)
+    want = workloads.planck_band64(Tnb.astype(np.float64)[None, :], nu[:, None])
+    np.testing.assert_allclose(ib, want, rtol=2e-5, atol=1e-30)
+    np.testing.assert_array_equal(S.array("kq"), np.ones(nB * nT, np.float32))
+    seed = cfg.seed
+    _check_rays_and_tallies(oracle, S, cfg, lambda P, c: oracle.gen_philox(P, seed, 0, c, 0, 32), 32)
+
+
+def test_grey_ref_tables_slab_closed_form(oracle):
+    """grey==1 on the isothermal slab: every band has kappa = 1, so the
+    wall-arrival fractions are the grey slab's (cut-off included) and
+    Q(y) = 4 pi sum_b dnu_b (Ib_b(T) - Ib_b(Tw)) F(y); the emission
+    4 pi sum_b dnu_b Ib_b(T) is 4 sigma T^4 to the table's interpolation."""
+    cfg = small_channel(n=(4, 32, 4), R=20000, spectral=1)
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    S.set_field(0, workloads.ghosted(np.full(tuple(cfg.n), 1000.0, np.float32)))
+    S.run()
+    r = S.results(0)
+    sc = S.array("scalars")
+    nB, nT = int(sc[3]), int(sc[5])
+    inv_dT, mT0 = np.float32(sc[6]), np.float32(sc[7])
+    pos = float(np.float32(np.float32(1000.0) * inv_dT + mT0))
+    t0 = min(int(pos), nT - 2)
+    fr = pos - t0
+    ib = S.array("Ib").reshape(nB, nT).astype(np.float64)
+    ib_gas = ib[:, t0] + fr * (ib[:, t0 + 1] - ib[:, t0])
+    ib_w = S.array("Ibw")[2 * nB:3 * nB].astype(np.float64)   # the y = 0 wall (axis 1, low side) at 300 K
+    np.testing.assert_allclose(ib_w, workloads.planck_band64(300.0, 15.0 * np.arange(nB) + 7.5), rtol=2e-5,
+                               atol=1e-30)
+    dnu = 1500.0
+    emis = 4.0 * math.pi * dnu * ib_gas.sum()
+    np.testing.assert_allclose(r["E"], emis, rtol=2e-5)
+    assert emis == pytest.approx(4.0 * 5.670373e-08 * 1000.0 ** 4, rel=1e-3)
+    fy = S.array("faces1", 0).astype(np.float64)
+    ny = cfg.n[1]
+    F = oracle.slab_exchange(fy, np.ones(ny), np.ones(ny), 0.0, 0.0, tau_cut=-math.log(1e-3), npanel=4000)
+    Qref = 4.0 * math.pi * dnu * (ib_gas - ib_w).sum() * F
+    Q = r["Q"].reshape(cfg.n[2], ny, cfg.n[0])
+    z = (Q - Qref[None, :, None]) / r["sdev"].reshape(Q.shape)
+    assert np.max(np.abs(z)) < 5.0
+    assert abs(np.mean(z)) < 0.5
+
+
 def test_legacy_abi_fortran_layout(tmp_path):
     """mc_gpu_/get_results_ through the Fortran-layout boundary."""
     conf = tmp_path / "rad.conf"

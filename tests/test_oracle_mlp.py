@@ -82,6 +82,73 @@ def test_dropped_term_is_caught(oracle):
     assert np.max(np.abs(gq[0] - g[0])) > 1e-3 * np.max(np.abs(g[0]))
 
 
+# ------------------------------------------- BF16 rounding mode (the kernels')
+def bf16_numpy(a):
+    """FP64 -> FP32 (nearest even) -> BF16 (nearest even on the FP32 bits),
+    vectorised on the integer bits."""
+    u = np.asarray(a, np.float64).astype(np.float32).view(np.uint32).astype(np.uint64)
+    finite = (u & 0x7F800000) != 0x7F800000
+    r = np.where(finite, (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000, u & 0xFFFF0000)
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_bf16_round_matches_numpy(oracle):
+    rng = np.random.default_rng(6)
+    x = np.concatenate([rng.standard_normal(2000) * 10.0 ** rng.integers(-6, 6, 2000),
+                        # exact ties of the FP32 -> BF16 step: round to even both ways
+                        np.array([1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -(1.0 + 2.0 ** -8), 0.0, -0.0,
+                                  2.0 ** -130, 3.4e38, np.inf, -np.inf])])
+    got = np.array([oracle.bf16_round(v) for v in x])
+    want = bf16_numpy(x)
+    np.testing.assert_array_equal(got, want)
+    assert oracle.bf16_round(1.0 + 2.0 ** -8) == 1.0                   # tie to even (down)
+    assert oracle.bf16_round(1.0 + 3 * 2.0 ** -8) == 1.0 + 2.0 ** -6   # tie to even (up)
+
+
+def _bf16_numpy_net(X, p, ndir, td, tb, wt):
+    """Forward and backward with the kernels' rounding points, in numpy."""
+    W1, b1, W2, b2, W3, b3 = p
+    H1 = np.tanh(X @ W1 + b1)
+    H1q = bf16_numpy(H1)
+    H2 = np.tanh(H1q @ W2 + b2)
+    H2q = bf16_numpy(H2)
+    Z = H2q @ W3 + b3
+    sc = wt / wt.sum()
+    dZ = np.zeros_like(Z)
+    for lo, hi, tgt in ((0, ndir, td), (ndir, Z.shape[1], tb)):
+        z = Z[:, lo:hi]
+        e = np.exp(z - z.max(1, keepdims=True))
+        sm = e / e.sum(1, keepdims=True)
+        sm[np.arange(len(tgt)), tgt] -= 1.0
+        dZ[:, lo:hi] = sc[:, None] * sm
+    dZq = bf16_numpy(dZ)
+    dA2 = (dZq @ W3.T) * (1.0 - H2 ** 2)
+    dA2q = bf16_numpy(dA2)
+    dA1 = (dA2q @ W2.T) * (1.0 - H1 ** 2)
+    g = [X.T @ dA1, dA1.sum(0), H1q.T @ dA2q, dA2.sum(0), H2q.T @ dZq, dZ.sum(0)]
+    return Z, g
+
+
+def test_bf16_mode_matches_numpy(oracle):
+    rng = np.random.default_rng(8)
+    p = _net(rng, nh=16, nout=29)
+    nb, ndir = 11, 7
+    X = rng.standard_normal((nb, 6))
+    td = rng.integers(0, ndir, nb)
+    tb = rng.integers(0, 29 - ndir, nb)
+    wt = rng.random(nb) + 0.1
+    Zn, gn = _bf16_numpy_net(X, p, ndir, td, tb, wt)
+    _, _, Z = oracle.mlp_forward(X, *p, bf16=True)
+    np.testing.assert_allclose(Z, Zn, rtol=1e-12, atol=1e-13)
+    _, g = oracle.mlp_loss_grad(X, *p, ndir, td, tb, wt, bf16=True)
+    for a, b in zip(g, gn):
+        np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-13)
+    # the rounding is there (the modes differ) and stays at BF16 scale
+    _, gf = oracle.mlp_loss_grad(X, *p, ndir, td, tb, wt)
+    rel = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(g, gf)]
+    assert max(rel) > 1e-5 and max(rel) < 2e-2, rel
+
+
 # ------------------------------------------------ learned draw (orc_learned_draw)
 def _rows(rng, nslots, ndir, nbin):
     """Unnormalised inclusive cumulative weights, as the proposal writes them."""
