@@ -23,6 +23,7 @@ EXPORTED = [
     "dnsmc_set_emitters_slab", "dnsmc_set_emitters_list", "dnsmc_run", "dnsmc_wait",
     "dnsmc_get_results", "dnsmc_get_results_fortran", "dnsmc_get_stats", "dnsmc_get_array",
     "dnsmc_get_rays", "dnsmc_get_ray_records", "dnsmc_set_record", "dnsmc_sync", "dnsmc_bench_copy",
+    "dnsmc_bench_stream",
     "dnsmc_mlp_nparams", "dnsmc_mlp_samples", "dnsmc_mlp_features", "dnsmc_mlp_grad", "dnsmc_mlp_adam",
     "dnsmc_mlp_forward", "dnsmc_dev_alloc", "dnsmc_dev_free", "dnsmc_dev_copy",
     "dnsmc_fix_temp_slab", "dnsmc_commit_field", "dnsmc_fix_rad_slab",
@@ -119,6 +120,7 @@ def lib():
     L.dnsmc_set_record.argtypes = [vp, i32]
     L.dnsmc_sync.argtypes = []
     L.dnsmc_bench_copy.argtypes = [C.c_longlong, i32, f64p]
+    L.dnsmc_bench_stream.argtypes = [C.c_longlong, i32, i32, f64p, C.POINTER(C.c_int)]
     ll, u16p = C.c_longlong, C.c_void_p
     L.dnsmc_mlp_nparams.argtypes = [i32]
     L.dnsmc_mlp_nparams.restype = ll

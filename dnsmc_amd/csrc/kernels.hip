@@ -1421,6 +1421,59 @@ __global__ void copy_f32x4(const float4* __restrict__ src, float4* __restrict__ 
         dst[i] = src[i];
 }
 
+// Streaming kernels of the achievable-bandwidth ceiling: every lane moves U
+// 16-byte vectors per iteration (grid-stride, loads issued before stores),
+// optionally with non-temporal hints.  stream_read sums what it reads and
+// writes one float per thread only when the sum is the impossible value -1,
+// so the loads are not removed and nothing else is written.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int U, bool NT>
+__global__ __launch_bounds__(256) void stream_copy(const float4* __restrict__ src4, float4* __restrict__ dst4,
+                                                   long long n4)
+{
+    const v4f* src = (const v4f*)src4;
+    v4f* dst = (v4f*)dst4;
+    const long long st = (long long)gridDim.x * blockDim.x;
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; i + (U - 1) * st < n4; i += U * st) {
+        v4f v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = NT ? __builtin_nontemporal_load(src + i + u * st) : src[i + u * st];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (NT) __builtin_nontemporal_store(v[u], dst + i + u * st);
+            else dst[i + u * st] = v[u];
+        }
+    }
+    for (; i < n4; i += st) dst[i] = src[i];
+}
+
+template <int U, bool NT>
+__global__ __launch_bounds__(256) void stream_read(const float4* __restrict__ src4, long long n4, float* sink)
+{
+    const v4f* src = (const v4f*)src4;
+    const long long st = (long long)gridDim.x * blockDim.x;
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    float acc = 0.f;
+    for (; i + (U - 1) * st < n4; i += U * st) {
+        v4f v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = NT ? __builtin_nontemporal_load(src + i + u * st) : src[i + u * st];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += (v[u].x + v[u].y) + (v[u].z + v[u].w);
+    }
+    for (; i < n4; i += st) acc += src[i].x;
+    if (acc == -1.0f) sink[blockIdx.x * (long long)blockDim.x + threadIdx.x] = acc;
+}
+
+template __global__ void stream_copy<1, false>(const float4*, float4*, long long);
+template __global__ void stream_copy<4, false>(const float4*, float4*, long long);
+template __global__ void stream_copy<4, true>(const float4*, float4*, long long);
+template __global__ void stream_read<1, false>(const float4*, long long, float*);
+template __global__ void stream_read<4, false>(const float4*, long long, float*);
+template __global__ void stream_read<4, true>(const float4*, long long, float*);
+
 // ------------------------------------------------- DNS coupling slabs ----
 // fix_temp (code/fix_var.f:49-73) for one rank's slab, written straight into
 // the bricked device field instead of summed into a full-size host buffer and

@@ -64,6 +64,10 @@ __global__ void slab_to_field(const double* cnew, int n0, int n1, int n2, int i0
 __global__ void field_to_slab(const int* emit, long long nemit, const double* Q, const double* sd, int n0, int n1,
                               int jlo, int nj, int k0, int nk, double stefan, double tc4, double* divq, double* vcore);
 __global__ void copy_f32x4(const float4* src, float4* dst, long long n4);
+template <int U, bool NT>
+__global__ void stream_copy(const float4* src, float4* dst, long long n4);
+template <int U, bool NT>
+__global__ void stream_read(const float4* src, long long n4, float* sink);
 }  // namespace dn
 
 using namespace dn;
@@ -1749,6 +1753,64 @@ int dnsmc_bench_copy(long long nbytes, int reps, double* gbps)
     (void)hipEventDestroy(e1);
     (void)hipFree(a);
     (void)hipFree(b);
+    return DNSMC_OK;
+}
+
+int dnsmc_bench_stream(long long nbytes, int reps, int kind, double* gbps, int* best)
+{
+    if (nbytes < 16 || reps < 1 || !gbps || kind < 0 || kind > 1) return fail(DNSMC_EINVAL, "bad arguments");
+    const long long n4 = nbytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    float* sink = nullptr;
+    HIPCHK(hipMalloc((void**)&a, n4 * 16));
+    if (kind == 0) HIPCHK(hipMalloc((void**)&b, n4 * 16));
+    int dev = 0, ncu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int per_cu[4] = {4, 8, 16, 32};
+    HIPCHK(hipMalloc((void**)&sink, (size_t)ncu * 32 * 256 * sizeof(float)));
+    HIPCHK(hipMemset(a, 0, n4 * 16));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    // the variants: 16 B x {1, 4} per lane and iteration, plain or non-temporal,
+    // 4 to 32 blocks of 256 threads per CU; the best rate is the ceiling
+    double top = 0.0;
+    int top_v = -1;
+    for (int v = 0; v < 12; ++v) {
+        const int shape = v % 3, nbk = ncu * per_cu[v / 3];
+        auto launch = [&]() {
+            if (kind == 0) {
+                if (shape == 0) hipLaunchKernelGGL((stream_copy<1, false>), dim3(nbk), dim3(256), 0, 0, a, b, n4);
+                else if (shape == 1) hipLaunchKernelGGL((stream_copy<4, false>), dim3(nbk), dim3(256), 0, 0, a, b, n4);
+                else hipLaunchKernelGGL((stream_copy<4, true>), dim3(nbk), dim3(256), 0, 0, a, b, n4);
+            } else {
+                if (shape == 0) hipLaunchKernelGGL((stream_read<1, false>), dim3(nbk), dim3(256), 0, 0, a, n4, sink);
+                else if (shape == 1) hipLaunchKernelGGL((stream_read<4, false>), dim3(nbk), dim3(256), 0, 0, a, n4, sink);
+                else hipLaunchKernelGGL((stream_read<4, true>), dim3(nbk), dim3(256), 0, 0, a, n4, sink);
+            }
+        };
+        launch();   // warm-up
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e0, 0));
+        for (int r = 0; r < reps; ++r) launch();
+        HIPCHK(hipEventRecord(e1, 0));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        const double g = (kind == 0 ? 2.0 : 1.0) * (double)n4 * 16.0 * reps / (ms * 1e-3) / 1e9;
+        if (g > top) {
+            top = g;
+            top_v = v;
+        }
+    }
+    *gbps = top;
+    if (best) *best = top_v;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    (void)hipFree(sink);
     return DNSMC_OK;
 }
 

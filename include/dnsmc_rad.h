@@ -306,6 +306,13 @@ int dnsmc_sync(void);
  * `nbytes` (kernel copy_f32x4); returns read+write GB/s.  Used as the
  * achievable-bandwidth reference and to calibrate HBM byte counters. */
 int dnsmc_bench_copy(long long nbytes, int reps, double *gbps);
+/* Measurement hook: the achievable HBM bandwidth, the best of twelve streaming
+ * variants (16 B per lane x 1 or 4 per iteration, plain or non-temporal, 4 to
+ * 32 blocks of 256 threads per CU) over `nbytes` (use several GiB), `reps`
+ * timed launches each.  kind 0: copy (read + write GB/s), kind 1: read only.
+ * best (may be NULL) receives the winning variant (index: per-CU blocks
+ * 4 << (v / 3), shape v % 3 = 1x16 B, 4x16 B, 4x16 B non-temporal). */
+int dnsmc_bench_stream(long long nbytes, int reps, int kind, double *gbps, int *best);
 
 #ifdef __cplusplus
 }
