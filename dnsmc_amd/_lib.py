@@ -20,7 +20,7 @@ EXPORTED = [
     "mc_gpu_", "get_results_", "mc_gpu", "get_results",
     "dnsmc_default_config", "dnsmc_parse_config", "dnsmc_create", "dnsmc_destroy", "dnsmc_last_error",
     "dnsmc_load_tables", "dnsmc_probe_tables", "dnsmc_set_tables", "dnsmc_set_field", "dnsmc_set_mlp",
-    "dnsmc_set_emitters_slab", "dnsmc_set_emitters_list", "dnsmc_run", "dnsmc_wait",
+    "dnsmc_set_emitters_slab", "dnsmc_set_emitters_list", "dnsmc_run", "dnsmc_run_noalloc", "dnsmc_wait",
     "dnsmc_get_results", "dnsmc_get_results_fortran", "dnsmc_get_stats", "dnsmc_get_array",
     "dnsmc_get_rays", "dnsmc_get_ray_records", "dnsmc_set_record", "dnsmc_sync", "dnsmc_bench_copy",
     "dnsmc_bench_stream",
@@ -108,6 +108,7 @@ def lib():
     L.dnsmc_set_emitters_slab.argtypes = [vp, i32]
     L.dnsmc_set_emitters_list.argtypes = [vp, C.POINTER(C.c_int), i32]
     L.dnsmc_run.argtypes = [vp]
+    L.dnsmc_run_noalloc.argtypes = [vp, vp, vp]
     L.dnsmc_wait.argtypes = [vp]
     L.dnsmc_get_results.argtypes = [vp, i32, f64p, f64p, f64p, f64p]
     L.dnsmc_get_results_fortran.argtypes = [vp, i32, f32p, f32p]

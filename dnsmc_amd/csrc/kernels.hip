@@ -1347,6 +1347,7 @@ __global__ void reduce_rays(const float2* pray, int tally, long long nslots, int
 // compat: nvar block estimates De_v = E_v - A_v; Q = mean; reference variance
 // sum (De_v - Q)^2 / ((n - 0.5)(n - 1)) (code/mc_gpu.cu:217-222), sdev = sqrt.
 // plain: Q = E - A, var(Q) = (R sum x^2 - Q^2)/(R - 1) from per-ray x = e - a.
+// Eo and Ao (the mean emitted and absorbed parts) may be null.
 __global__ void finalize(const double* E, const double* A, const double* Q2, long long nemit, int spe, int R,
                          int compat, double* Q, double* Eo, double* Ao, double* sd)
 {
@@ -1365,11 +1366,15 @@ __global__ void finalize(const double* E, const double* A, const double* Q2, lon
             double d = (E[e * spe + v] - A[e * spe + v]) - m;
             var += 1.0 / (spe - 0.5) * 1.0 / (spe - 1) * d * d;
         }
-        Q[e] = m; Eo[e] = me; Ao[e] = ma; sd[e] = sqrt(var);
+        Q[e] = m; sd[e] = sqrt(var);
+        if (Eo) Eo[e] = me;
+        if (Ao) Ao[e] = ma;
     } else {
         double q = E[e] - A[e];
         double v = R > 1 ? ((double)R * Q2[e] - q * q) / (double)(R - 1) : 0.0;
-        Q[e] = q; Eo[e] = E[e]; Ao[e] = A[e]; sd[e] = sqrt(v > 0 ? v : 0.0);
+        Q[e] = q; sd[e] = sqrt(v > 0 ? v : 0.0);
+        if (Eo) Eo[e] = E[e];
+        if (Ao) Ao[e] = A[e];
     }
 }
 

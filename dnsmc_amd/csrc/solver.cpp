@@ -87,6 +87,45 @@ static int fail(int code, const std::string& msg)
             return fail(DNSMC_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));    \
     } while (0)
 
+// Allocation-free solve (dnsmc_run_noalloc): the fused traversal's tallies,
+// work-claim counters and step counter live in this file-scope device array,
+// not in solver buffers.  Each device holds its own copy for the life of the
+// process.  Host code addresses it only through the per-device pointer that
+// hipGetSymbolAddress returns (scratch_base), never through the array's name.
+// Layout: bytes [0, 1024) the 8 per-XCD claim counters (32 words apart),
+// [1024, 1032) the step counter, from byte 4096 the FP64 tallies E, A and
+// sum x^2, kScratchSlots slot-snapshots each.
+constexpr long long kScratchBytes = 64ll << 20;
+constexpr long long kScratchHead = 4096;
+constexpr long long kScratchSlots = (kScratchBytes - kScratchHead) / (3 * (long long)sizeof(double));
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch[kMaxDevices] = {};
+
+// The scratch array's address on device `dev` (current), looked up once.
+static hipError_t scratch_base(int dev, char** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev] = (char*)ptr;
+    }
+    *out = g_scratch[dev];
+    return hipSuccess;
+}
+
+// Where an allocation-free run keeps its counters and tallies and writes its
+// results (device pointers).
+struct NoAlloc {
+    char* base;          // this device's scratch array
+    double* Q;           // caller's results [nsnap][nemit]
+    double* sd;
+};
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -98,6 +137,7 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    bool fits(size_t cnt) const { return p != nullptr && cnt <= n; }
     hipError_t ensure(size_t cnt)
     {
         if (cnt <= n && p) return hipSuccess;
@@ -213,7 +253,29 @@ struct dnsmc_solver {
     long long last_nemit = 0;
     int ystart = 1;
     int last_ystart = 1;   // emitter j slab of the last run (its results' placement)
+    char* scratch = nullptr;                       // this device's allocation-free scratch array
+    unsigned long long* nsteps_src = nullptr;      // step counter of the last run
 };
+
+// Grow a solver buffer, or, in an allocation-free run, check that it already
+// holds what the run needs.
+#define NEED(buf, cnt)                                                                                   \
+    do {                                                                                                 \
+        if (na) {                                                                                        \
+            if (!(buf).fits((size_t)(cnt)))                                                              \
+                return fail(DNSMC_ESTATE, std::string("dnsmc_run_noalloc: ") + #buf +                    \
+                                              " would have to grow: run dnsmc_run once with this "       \
+                                              "configuration, emitters and temperature range first");    \
+        } else                                                                                           \
+            HIPCHK((buf).ensure((size_t)(cnt)));                                                         \
+    } while (0)
+#define PUT(buf, vec)                                                                                    \
+    do {                                                                                                 \
+        NEED(buf, (vec).size());                                                                         \
+        if (!(vec).empty())                                                                              \
+            HIPCHK(hipMemcpyAsync((buf).p, (vec).data(), (vec).size() * sizeof((vec)[0]),               \
+                                  hipMemcpyHostToDevice, s->stream));                                    \
+    } while (0)
 
 // ------------------------------------------------------------------ util ----
 static int spe_of(const dnsmc_config& c) { return c.mode == DNSMC_MODE_COMPAT ? c.nvar : 1; }
@@ -397,7 +459,7 @@ static int run_spline(dnsmc_solver* s, int snap)
 }
 
 // ------------------------------------------------------------ tables ----
-static int ensure_derived(dnsmc_solver* s, float Tmax)
+static int ensure_derived(dnsmc_solver* s, float Tmax, const NoAlloc* na)
 {
     if (s->der_Tmax == Tmax) return DNSMC_OK;
     if (!s->have_tables) return fail(DNSMC_ESTATE, "spectral tables not loaded (dnsmc_load_tables)");
@@ -406,13 +468,13 @@ static int ensure_derived(dnsmc_solver* s, float Tmax)
     int rc = build_derived(s->cfg, s->tab, Tmax, s->der, &err);
     if (rc) return fail(rc, err);
     const Derived& d = s->der;
-    HIPCHK(s->d_kq.upload(d.kq, s->stream));
-    HIPCHK(s->d_Ib.upload(d.Ib, s->stream));
-    HIPCHK(s->d_Ibw.upload(d.Ibw, s->stream));
-    HIPCHK(s->d_cdfb.upload(d.cdf_band, s->stream));
-    HIPCHK(s->d_cdfg.upload(d.cdf_g, s->stream));
-    HIPCHK(s->d_inv_emax.upload(d.inv_emax, s->stream));
-    if (d.sorted) HIPCHK(s->d_band_order.upload(d.band_order, s->stream));
+    PUT(s->d_kq, d.kq);
+    PUT(s->d_Ib, d.Ib);
+    PUT(s->d_Ibw, d.Ibw);
+    PUT(s->d_cdfb, d.cdf_band);
+    PUT(s->d_cdfg, d.cdf_g);
+    PUT(s->d_inv_emax, d.inv_emax);
+    if (d.sorted) PUT(s->d_band_order, d.band_order);
     {   // plain probability of each (band, g) bin: the CDF steps the plain draw uses
         std::vector<float> pbg((size_t)d.nB * d.nQ, 0.f);
         double pb0 = 0.0;
@@ -426,7 +488,7 @@ static int ensure_derived(dnsmc_solver* s, float Tmax)
                 pbg[(size_t)b * d.nQ + g] = (float)(pb * pg);
             }
         }
-        HIPCHK(s->d_pbg.upload(pbg, s->stream));
+        PUT(s->d_pbg, pbg);
         std::vector<float> pcdf(pbg.size());   // inclusive CDF of pbg, last entry exactly 1
         double acc = 0.0;
         for (size_t i = 0; i < pbg.size(); ++i) {
@@ -434,7 +496,7 @@ static int ensure_derived(dnsmc_solver* s, float Tmax)
             pcdf[i] = (float)std::min(acc, 1.0);
         }
         if (!pcdf.empty()) pcdf.back() = 1.0f;
-        HIPCHK(s->d_pcdf.upload(pcdf, s->stream));
+        PUT(s->d_pcdf, pcdf);
     }
     s->der_Tmax = Tmax;
     return DNSMC_OK;
@@ -607,11 +669,23 @@ static FeatArgs feat_args(dnsmc_solver* s, const Derived& d)
     return f;
 }
 
-static int do_run(dnsmc_solver* s)
+// The whole solve, queued.  With `na` (dnsmc_run_noalloc) nothing is
+// allocated: the fused plain path only, counters and tallies in the scratch
+// array, results into the caller's arrays, no timing events.
+static int do_run(dnsmc_solver* s, const NoAlloc* na)
 {
     const dnsmc_config& c = s->cfg;
     int rc = check_config(c);
     if (rc) return rc;
+    if (na) {
+        if (c.mode != DNSMC_MODE_PLAIN || c.precision != 0 || c.tally != DNSMC_TALLY_F64_ATOMIC || s->record)
+            return fail(DNSMC_EINVAL, "dnsmc_run_noalloc runs the plain estimator only (mode 0, precision 0, "
+                                      "tally 0, records off)");
+        if ((long long)s->emit.size() * c.nsnap > kScratchSlots)
+            return fail(DNSMC_EINVAL, "dnsmc_run_noalloc: " + std::to_string((long long)s->emit.size()) +
+                                          " emitters x " + std::to_string(c.nsnap) + " snapshots exceed the " +
+                                          std::to_string(kScratchSlots) + " tally slots of its 64 MiB scratch");
+    }
     for (int i = 0; i < c.nsnap; ++i)
         if (!s->field_set[i]) return fail(DNSMC_ESTATE, "temperature field of snapshot " + std::to_string(i) + " not set");
     if (s->emit.empty()) return fail(DNSMC_ESTATE, "no emitting cells");
@@ -623,7 +697,7 @@ static int do_run(dnsmc_solver* s)
     for (int i = 0; i < c.nsnap; ++i) Tmax = std::max(Tmax, s->Tmax_snap[i]);
     for (int a = 0; a < 3; ++a)
         for (int sd = 0; sd < 2; ++sd) Tmax = std::max(Tmax, c.Tw[a][sd]);   // code/mc_gpu.cu:481-486
-    rc = ensure_derived(s, Tmax);
+    rc = ensure_derived(s, Tmax, na);
     if (rc) return rc;
     if (c.mode == DNSMC_MODE_COMPAT && (rc = ensure_compat(s))) return rc;
     const int spe = spe_of(c), R = rays_per_slot_of(c);
@@ -631,9 +705,9 @@ static int do_run(dnsmc_solver* s)
         brick_order(s->emit, c.n[0], c.n[1], c.n[2], s->order);
         std::vector<int4> si;
         build_slotinfo(s->emit, s->order, spe, c.n, s->nbx[0], s->nby[0], si);
-        HIPCHK(s->d_emit.upload(s->emit, s->stream));
-        HIPCHK(s->d_order.upload(s->order, s->stream));
-        HIPCHK(s->d_slotinfo.upload(si, s->stream));
+        PUT(s->d_emit, s->emit);
+        PUT(s->d_order, s->order);
+        PUT(s->d_slotinfo, si);
         s->emit_dirty = false;
     }
     const Derived& d = s->der;
@@ -672,7 +746,7 @@ static int do_run(dnsmc_solver* s)
     // more than 2^32 - 2^20 rays per snapshot take several launches over
     // consecutive slot ranges (ray indices within a launch are 32-bit)
     bool fused = c.mode == DNSMC_MODE_PLAIN && c.precision == 0 && c.tally == DNSMC_TALLY_F64_ATOMIC && !s->record;
-    if (const char* fe = getenv("DNSMC_FUSED")) fused = fused && atoi(fe) != 0;
+    if (const char* fe = getenv("DNSMC_FUSED")) fused = fused && (atoi(fe) != 0 || na);
     if (fused) slots_per_batch = nslots;
     // learned estimator, atomic tallies, no records: the descriptors of a whole
     // snapshot are generated (batch by batch, on a second stream) before one
@@ -696,7 +770,7 @@ static int do_run(dnsmc_solver* s)
     if (const char* pe = getenv("DNSMC_PIPES")) pipes_max = std::max(1, std::min(dnsmc_solver::kPipes, atoi(pe)));
     // the snapshot-wise path always uses streams 0 (traversal) and 1 (generation)
     const int npipes = snapwise ? 2 : (int)std::min<long long>(pipes_max, nbatch_total);
-    for (int p = 0; p < npipes; ++p) {
+    for (int p = 0; p < npipes && !na; ++p) {
         dnsmc_solver::Pipe& pp = s->pipe[p];
         // one snapshot needs only descriptor set 0
         const long long nset = snapwise && p == 1 && c.nsnap < 2 ? 0 : batch_rays;
@@ -710,14 +784,31 @@ static int do_run(dnsmc_solver* s)
             HIPCHK(pp.d_qcdf.ensure((size_t)std::min(slots_per_batch, nslots) * mlp_noutp(s->mlp_nbin)));
         HIPCHK(pp.d_claim.ensure(256));
     }
-    HIPCHK(s->d_E.ensure(nslots * c.nsnap));
-    HIPCHK(s->d_A.ensure(nslots * c.nsnap));
-    HIPCHK(s->d_Q2.ensure(nslots * c.nsnap));
-    HIPCHK(s->d_Q.ensure(nemit * c.nsnap));
-    HIPCHK(s->d_Eo.ensure(nemit * c.nsnap));
-    HIPCHK(s->d_Ao.ensure(nemit * c.nsnap));
-    HIPCHK(s->d_sd.ensure(nemit * c.nsnap));
-    HIPCHK(s->d_nsteps.ensure(1));
+    // tallies, results and the step counter: solver buffers, or the scratch
+    // array and the caller's arrays (allocation-free run)
+    double *tE, *tA, *tQ2;
+    unsigned long long* nsteps;
+    unsigned* claim0 = nullptr;   // allocation-free run: its claim counters
+    if (na) {
+        claim0 = (unsigned*)na->base;
+        nsteps = (unsigned long long*)(na->base + 1024);
+        tE = (double*)(na->base + kScratchHead);
+        tA = tE + kScratchSlots;
+        tQ2 = tA + kScratchSlots;
+    } else {
+        HIPCHK(s->d_E.ensure(nslots * c.nsnap));
+        HIPCHK(s->d_A.ensure(nslots * c.nsnap));
+        HIPCHK(s->d_Q2.ensure(nslots * c.nsnap));
+        HIPCHK(s->d_Q.ensure(nemit * c.nsnap));
+        HIPCHK(s->d_Eo.ensure(nemit * c.nsnap));
+        HIPCHK(s->d_Ao.ensure(nemit * c.nsnap));
+        HIPCHK(s->d_sd.ensure(nemit * c.nsnap));
+        HIPCHK(s->d_nsteps.ensure(1));
+        tE = s->d_E.p;
+        tA = s->d_A.p;
+        tQ2 = s->d_Q2.p;
+        nsteps = s->d_nsteps.p;
+    }
     if (s->record) {
         s->nrec = std::min<long long>(batch_rays, 1ll << 22);
         HIPCHK(s->d_rsteps.ensure(s->nrec));
@@ -730,13 +821,13 @@ static int do_run(dnsmc_solver* s)
     memset(&s->st, 0, sizeof s->st);
     s->t0 = std::chrono::steady_clock::now();
     // counters are cleared by fill kernels (stream-ordered, capture-safe)
-    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s->stream, (unsigned*)s->d_nsteps.p, 2ll, 0u);
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s->stream, (unsigned*)nsteps, 2ll, 0u);
     {
         long long n = nslots * c.nsnap;
         int nb = (int)((n + 255) / 256);
-        hipLaunchKernelGGL(fill_f64, dim3(nb), dim3(256), 0, s->stream, s->d_E.p, n, 0.0);
-        hipLaunchKernelGGL(fill_f64, dim3(nb), dim3(256), 0, s->stream, s->d_A.p, n, 0.0);
-        hipLaunchKernelGGL(fill_f64, dim3(nb), dim3(256), 0, s->stream, s->d_Q2.p, n, 0.0);
+        hipLaunchKernelGGL(fill_f64, dim3(nb), dim3(256), 0, s->stream, tE, n, 0.0);
+        hipLaunchKernelGGL(fill_f64, dim3(nb), dim3(256), 0, s->stream, tA, n, 0.0);
+        hipLaunchKernelGGL(fill_f64, dim3(nb), dim3(256), 0, s->stream, tQ2, n, 0.0);
     }
     HIPCHK(hipEventRecord(s->ev_ready, s->stream));
     for (int p = 1; p < npipes; ++p) HIPCHK(hipStreamWaitEvent(s->pipe[p].stream, s->ev_ready, 0));
@@ -816,7 +907,7 @@ static int do_run(dnsmc_solver* s)
                 // slice instead of its temperature (t_to_u), every level and snapshot
                 for (size_t l = 0; l < s->lev.size(); ++l) {
                     const long long nl = s->tstride[l] * c.nsnap;
-                    HIPCHK(s->d_U[l].ensure(nl));
+                    NEED(s->d_U[l], nl);
                     const int nb = (int)std::min<long long>((nl + 255) / 256, (long long)s->ncu * 16);
                     hipLaunchKernelGGL(t_to_u, dim3(nb), dim3(256), 0, s->stream, (const float*)s->d_T[l].p,
                                        s->d_U[l].p, nl, d.inv_dT, d.mT0, (float)(d.nT - 1), (float)lo);
@@ -863,7 +954,7 @@ static int do_run(dnsmc_solver* s)
     }
     P.slotinfo = s->d_slotinfo.p;
     P.tally = c.tally;
-    P.nsteps = s->d_nsteps.p;
+    P.nsteps = nsteps;
     P.nrec = s->record ? s->nrec : 0;
     P.rec_steps = s->d_rsteps.p; P.rec_end = s->d_rend.p; P.rec_hash = s->d_rhash.p;
     P.rec_D = s->d_rD.p; P.rec_G = s->d_rG.p;
@@ -942,24 +1033,24 @@ static int do_run(dnsmc_solver* s)
         P.g_search = d.g_search;
         P.w0 = nullptr; P.w1 = nullptr; P.w = nullptr; P.winv = nullptr;
         P.pray = nullptr;
-        P.E = s->d_E.p;
-        P.A = s->d_A.p;
-        P.Q2 = s->d_Q2.p;
+        P.E = tE;
+        P.A = tA;
+        P.Q2 = tQ2;
         P.nrec = 0;
-        P.claim = pp.d_claim.p;
+        P.claim = na ? claim0 : pp.d_claim.p;
         long long part = std::max<long long>(1, ((1ll << 32) - (1ll << 20)) / R);   // slots per launch
         if (const char* fp = getenv("DNSMC_FUSED_SLOTS")) part = std::max<long long>(1, std::min(part, atoll(fp)));
         for (long long p0 = 0; p0 < nslots; p0 += part) {
             P.slot0 = p0;
             P.nrays = std::min(part, nslots - p0) * R;
-            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, pp.d_claim.p, 256ll, 0u);
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, P.claim, 256ll, 0u);
             HIPCHK(hipGetLastError());
             const long long nsc = ((P.nrays + P.super - 1) / P.super) * c.nsnap;
             const int tb_cap = (int)std::min<long long>(trace_blocks, std::max<long long>(8, (nsc + 15) / 16));
-            EventPair& et = next_event(s, 1);
-            HIPCHK(hipEventRecord(et.a, st));
+            EventPair* et = na ? nullptr : &next_event(s, 1);
+            if (et) HIPCHK(hipEventRecord(et->a, st));
             HIPCHK(launch_trace(spec, tab, true, false, false, amc, s->ncu, tb_cap, lds_trace, st, P));
-            HIPCHK(hipEventRecord(et.b, st));
+            if (et) HIPCHK(hipEventRecord(et->b, st));
             s->st.n_trace_launches++;
         }
     }
@@ -1100,11 +1191,11 @@ static int do_run(dnsmc_solver* s)
     }
     for (int snap = 0; snap < c.nsnap; ++snap) {
         int nb = (int)((nemit + 255) / 256);
-        hipLaunchKernelGGL(finalize, dim3(nb), dim3(256), 0, s->stream, s->d_E.p + (size_t)snap * nslots,
-                           s->d_A.p + (size_t)snap * nslots, s->d_Q2.p + (size_t)snap * nslots, nemit, spe, R,
-                           c.mode == DNSMC_MODE_COMPAT ? 1 : 0, s->d_Q.p + (size_t)snap * nemit,
-                           s->d_Eo.p + (size_t)snap * nemit, s->d_Ao.p + (size_t)snap * nemit,
-                           s->d_sd.p + (size_t)snap * nemit);
+        const size_t o = (size_t)snap * nemit;
+        hipLaunchKernelGGL(finalize, dim3(nb), dim3(256), 0, s->stream, tE + (size_t)snap * nslots,
+                           tA + (size_t)snap * nslots, tQ2 + (size_t)snap * nslots, nemit, spe, R,
+                           c.mode == DNSMC_MODE_COMPAT ? 1 : 0, na ? na->Q + o : s->d_Q.p + o,
+                           na ? nullptr : s->d_Eo.p + o, na ? nullptr : s->d_Ao.p + o, na ? na->sd + o : s->d_sd.p + o);
         HIPCHK(hipGetLastError());
     }
     s->st.n_rays = nslots * (long long)R * c.nsnap;
@@ -1117,8 +1208,9 @@ static int do_run(dnsmc_solver* s)
     s->st.Ibmax = d.Ibmax_d;
     s->last_R = R;
     s->last_spe = spe;
-    s->last_nemit = nemit;
+    s->last_nemit = na ? 0 : nemit;   // an allocation-free run's results are the caller's only
     s->last_ystart = s->ystart;
+    s->nsteps_src = nsteps;
     s->running = true;
     return DNSMC_OK;
 }
@@ -1138,7 +1230,7 @@ static int do_wait(dnsmc_solver* s)
         else s->st.ms_other += ms;
     }
     unsigned long long ns = 0;
-    HIPCHK(hipMemcpy(&ns, s->d_nsteps.p, sizeof ns, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&ns, s->nsteps_src, sizeof ns, hipMemcpyDeviceToHost));
     s->st.n_steps = (long long)ns;
     return DNSMC_OK;
 }
@@ -1242,6 +1334,13 @@ int dnsmc_create(const dnsmc_config* cfg, dnsmc_solver** out)
     rc = build_levels(s->cfg, s->lev, &err);
     if (rc) { delete s; return fail(rc, err); }
     if (hipGetDevice(&s->dev) != hipSuccess) { delete s; return fail(DNSMC_EDEVICE, "no HIP device"); }
+    {   // the allocation-free run's scratch array on this device (looked up once per device)
+        const hipError_t e = scratch_base(s->dev, &s->scratch);
+        if (e != hipSuccess) {
+            delete s;
+            return fail(DNSMC_EDEVICE, std::string("scratch array address: ") + hipGetErrorString(e));
+        }
+    }
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->dev) == hipSuccess && ncu > 0) s->ncu = ncu;
     int lmax = 0;
@@ -1538,7 +1637,18 @@ int dnsmc_run(dnsmc_solver* s)
         int rc = do_wait(s);
         if (rc) return rc;
     }
-    return do_run(s);
+    return do_run(s, nullptr);
+}
+
+int dnsmc_run_noalloc(dnsmc_solver* s, double* Q, double* sdev)
+{
+    if (!s || !Q || !sdev) return fail(DNSMC_EINVAL, "null argument");
+    if (s->running) {
+        int rc = do_wait(s);
+        if (rc) return rc;
+    }
+    NoAlloc na{s->scratch, Q, sdev};
+    return do_run(s, &na);
 }
 
 int dnsmc_wait(dnsmc_solver* s)

@@ -239,6 +239,25 @@ int dnsmc_set_emitters_list(dnsmc_solver *s, const int *cells, int ncells);
  *   DNSMC_CDF_GB=g    GB of learned CDF rows per pipeline (default 8, or 32
  *                     when the device memory holds it) */
 int dnsmc_run(dnsmc_solver *s);
+/* Allocation-free variant of dnsmc_run.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It queues
+ * the plain estimator's solve of every snapshot exactly as dnsmc_run does
+ * (rays drawn inside the traversal; mode 0, precision 0, tally 0, records
+ * off, else DNSMC_EINVAL) and calls no hipMalloc, hipFree or other allocating
+ * routine: the tallies, work-claim counters and step counter live in a static
+ * 64 MiB device array of the library (one per GPU, held until the process
+ * ends), which limits a call to 2,796,032 emitters x snapshots (DNSMC_EINVAL
+ * beyond).  Results go to the caller's device arrays Q[nsnap][nemit] and
+ * sdev[nsnap][nemit] (FP64, emitter order), written when the solve completes
+ * (dnsmc_wait); dnsmc_get_results then has nothing to return.  Every other
+ * device buffer the solve reads must already exist: the field (dnsmc_set_field),
+ * and the slot, table and table-coordinate buffers that an earlier dnsmc_run
+ * with this configuration and at least as many emitters sized; otherwise
+ * DNSMC_ESTATE names the buffer.  All calls on one GPU share its scratch
+ * array, so two allocation-free solves that overlap on one GPU (other solvers,
+ * other streams, other threads) race and must be serialised by the caller. */
+int dnsmc_run_noalloc(dnsmc_solver *s, double *Q, double *sdev);
 /* Wait for the queued solve. */
 int dnsmc_wait(dnsmc_solver *s);
 /* Results of snapshot `snap` for the emitters, in emitter order (FP64):
