@@ -7,14 +7,18 @@ resident in HBM.  Default workload: ck384x8 = 384x256x256 channel, 8
 temperature snapshots (profile + Fourier modes) in one launch, synthetic
 100x16x69 correlated-k table, 100 rays/cell (2.0e10 rays per step).
 
-Prints ONE JSON line with rays/s (whole job), the traversal kernel's
-algorithmic HBM bandwidth against the MI355X roofline (with the HBM traffic
-and L2 hit rate measured by rocprof for the same command, profiles/), the
-CPU baseline (the oracle's restatement of the reference algorithm on this
-host's cores, timed on a bounded sample of emitting cells), and the learned
-proposal against the plain estimator: RMS / max error of Q_rad against a
-converged FP64 reference and the time to 1% RMS, on the fine grid and with
-the reference's default mesh coarsening.
+Prints ONE JSON line: rays/s (whole job); the traversal kernel's algorithmic
+HBM bandwidth against the MI355X peak and against the achievable streaming
+bandwidth measured in the same run, with the HBM traffic and L2 hit rate
+rocprof measured for the same command (profiles/); the CPU baseline (the
+oracle's restatement of the reference algorithm on this host's cores: a
+bounded sample of the workload, and full runs of the two small
+configurations); the accuracy legs (plain and learned estimators, fine grid
+and the reference's default coarsening, all against one converged fine-grid
+FP64 reference): RMS / max error, rays per cell to 1% RMS measured by
+doubling, the coarsening's bias and the wall-clock to 1% RMS; and one key per
+other configuration (grey 32^3, 64x96x64, 256x128x128 with its precision
+study).
 """
 import argparse
 import json
@@ -97,9 +101,38 @@ def cpu_baseline(S, cfg, fields, R, budget_s):
     _, _, _, nsteps = o.mc_philox(P, cfg.seed, 0, R, cells=cells, nthreads=threads)
     dt = time.time() - t0
     rays = k2 * R
-    return dict(value=rays / dt, unit="rays/s", cores=threads, kind="port",
-                sample="%d random emitting cells of snapshot 0 x %d rays (%.1f s, %.3g segments/s)"
-                       % (k2, R, dt, nsteps / dt))
+    out = dict(value=rays / dt, unit="rays/s", cores=threads, kind="port",
+               sample="%d random emitting cells of snapshot 0 x %d rays (%.1f s, %.3g segments/s)"
+                      % (k2, R, dt, nsteps / dt))
+    out["full_runs"] = cpu_full_runs(o, threads)
+    return out
+
+
+def cpu_full_runs(o, threads):
+    """The oracle over every cell of the grey 32^3 (100 rays/cell) and the
+    64x96x64 correlated-k (500 rays/cell) workloads on this host's cores, and
+    the largest relative difference of its emitted energies from the GPU
+    solve of the same rays (the estimator's arithmetic, FP32 rays)."""
+    from dnsmc_amd import study, workloads
+    out = {}
+    for wl in ("grey32", "ck64"):
+        R = workloads.WORKLOADS[wl]["rays"]
+        fields = workloads.workload_fields(wl, 1)
+        S, cfg = study.make_solver(wl, R, fields=fields)
+        S.run()
+        r = S.results(0)
+        P = o.problem(oracle_problem(S, cfg, fields[0]))
+        S.close()
+        t0 = time.time()
+        E, A, _, ns = o.mc_philox(P, cfg.seed, 0, R, nthreads=threads)
+        dt = time.time() - t0
+        ncell = int(np.prod(cfg.n))
+        out[wl] = dict(value=ncell * R / dt, unit="rays/s", cores=threads, seconds=round(dt, 2),
+                       rays=ncell * R, segments_per_s=ns / dt,
+                       max_rel_diff_E_vs_gpu=float(np.max(np.abs(E - r["E"]) / np.abs(r["E"]))),
+                       max_rel_diff_A_vs_gpu=float(np.max(np.abs(A - r["A"]) / np.abs(r["A"]))))
+        log("cpu full", wl, out[wl])
+    return out
 
 
 def measured_traffic(args, cfg, R):
@@ -136,76 +169,183 @@ def timed_solve(S, steps):
     return (time.perf_counter() - t0) / steps, st_all
 
 
-def learned_leg(args, cfg, R, fields, res_p, t_plain, cfg_kw):
-    """Train the proposal, run the learned solve of the workload, and compare
-    both estimators with a converged FP64 reference on 1024 cells of snapshot
-    0: RMS / max error at R rays per cell and the time to 1% RMS."""
+def solve_cost(workload, R, fields, mode=0, mlp=None, mix_eps=0.1, **cfg_kw):
+    """One full-grid solve of snapshot 0 at R and at 4R rays per cell (each
+    after a warm-up solve): the fixed part a and the per-ray-per-cell part b
+    of t(R) = a + b R, the two times, and the stats and results of the R solve."""
+    from dnsmc_amd import study
+    ts = []
+    keep = None
+    for RR in (R, 4 * R):
+        S1, _ = study.make_solver(workload, RR, mode=mode, mlp=mlp, mix_eps=mix_eps, fields=fields[:1], **cfg_kw)
+        S1.run()
+        S1.wait()
+        t0 = time.perf_counter()
+        S1.run()
+        S1.wait()
+        ts.append(time.perf_counter() - t0)
+        if keep is None:
+            keep = (S1.stats(), S1.results(0))
+        S1.close()
+    b = (ts[1] - ts[0]) / (3 * R)
+    return ts[0] - b * R, b, ts, keep[0], keep[1]
+
+
+def accuracy_legs(args, cfg, R, fields):
+    """Plain and learned estimators on the fine grid and with the reference's
+    default mesh coarsening, all scored against ONE converged FP64 reference
+    on the fine grid (1024 sampled cells of snapshot 0), so the coarsening's
+    interpolation bias counts: RMS / max error at the bench's R, rays per cell
+    to 1% RMS measured by doubling R, and the wall-clock of a full solve at
+    that R (the workload's snapshots, fixed + per-ray cost measured on the
+    full grid)."""
     from dnsmc_amd import learned, study
-    cells = study.sample_cells(args.workload, args.ref_cells)
-    Qref, vref, t_ref = study.reference(args.workload, cells, args.ref_rays, fields=fields, **cfg_kw)
-    weights, info = learned.train_channel_proposal(args.train_snaps, args.train_rays, args.cost0, args.train_epochs,
-                                                   **cfg_kw)
-    log("proposal: %s" % (info,))
-    SL, _ = study.make_solver(args.workload, R, nsnap=cfg.nsnap, mode=2, mlp=weights, mix_eps=args.mix_eps,
-                              fields=fields, **cfg_kw)
-    dt_l, st_l = timed_solve(SL, 1)
-    res_l = SL.results(0)
-    SL.close()
-    err_p = study.errors(res_p["Q"][cells], Qref, vref)
-    err_l = study.errors(res_l["Q"][cells], Qref, vref)
-    eps_p, eps_l = err_p["rms"], err_l["rms"]
-    t_p = t_plain * (eps_p / 0.01) ** 2
-    t_l = dt_l * (eps_l / 0.01) ** 2
-    var_ratio = float(np.sum(res_p["sdev"] ** 2) / np.sum(res_l["sdev"] ** 2))
-    out = {}
-    out["learned"] = dict(info, mix_eps=args.mix_eps, ms_per_step=dt_l * 1000.0,
-                          rays_per_s=st_l[0]["n_rays"] / dt_l,
-                          segments_per_ray=st_l[0]["n_steps"] / max(st_l[0]["n_rays"], 1),
-                          variance_ratio_vs_plain=var_ratio)
-    out["rms_error_vs_fp64_oracle"] = dict(
-        plain=eps_p, learned=eps_l, max_plain=err_p["max"], max_learned=err_l["max"], rays_per_cell=R,
-        snapshot=0, cells=int(len(cells)),
-        reference=dict(rays_per_cell=args.ref_rays, seconds=round(t_ref, 1),
-                       kind="converged FP64 MC (FP64 traversal and tallies, other seed), the oracle's estimator"),
-        estimate_from_variance=dict(plain=math.sqrt(np.sum(res_p["sdev"] ** 2) / np.sum(res_p["Q"] ** 2)),
-                                    learned=math.sqrt(np.sum(res_l["sdev"] ** 2) / np.sum(res_l["Q"] ** 2))))
-    # rays per cell for 1% RMS (error ~ R^-1/2) and the cost of a solve as a
-    # function of R: fixed part (per-cell proposal, launch, tails) plus a
-    # marginal cost per ray per cell, measured on snapshot 0 at R and 4R and
-    # scaled to the workload's snapshots
-    R1_p = R * (eps_p / 0.01) ** 2
-    R1_l = R * (eps_l / 0.01) ** 2
-    f0 = fields[:1]
-    lin = {}
-    for name, mode, W in (("plain", 0, None), ("learned", 2, weights)):
-        ts = []
-        for RR in (R, 4 * R):
-            S1, _ = study.make_solver(args.workload, RR, mode=mode, mlp=W, mix_eps=args.mix_eps, fields=f0, **cfg_kw)
-            S1.run()
-            S1.wait()
-            reps = []
-            for _ in range(3):   # median of three solves
-                t0 = time.perf_counter()
-                S1.run()
-                S1.wait()
-                reps.append(time.perf_counter() - t0)
-            ts.append(float(np.median(reps)))
-            S1.close()
-        b = (ts[1] - ts[0]) / (3 * R)
-        a = ts[0] - b * R
-        lin[name] = (a, b, ts)
-    ns = cfg.nsnap
-    T1_p = ns * (lin["plain"][0] + lin["plain"][1] * R1_p)
-    T1_l = ns * (lin["learned"][0] + lin["learned"][1] * R1_l)
+    wl = args.workload
+    cells = study.sample_cells(wl, args.ref_cells)
+    Qref, vref, t_ref = study.reference(wl, cells, args.ref_rays, fields=fields)
+    ref_noise = math.sqrt(float(np.sum(vref)) / float(np.sum(Qref ** 2)))
+    log("reference: %d rays/cell on %d cells, %.1f s, RMS noise %.4f" % (args.ref_rays, len(cells), t_ref, ref_noise))
+    amc = dict(grid_num=5, maxs=(5, 5, 5, 5, 100000, 100000))
+    legs = {}
+    weights = {}
+    for grid, kw in (("fine", {}), ("amc", amc)):
+        W, info = learned.train_channel_proposal(args.train_snaps, args.train_rays, args.cost0, args.train_epochs, **kw)
+        weights[grid] = W
+        log("proposal (%s): %s" % (grid, info))
+        for est, mode in (("plain", 0), ("learned", 2)):
+            name = "%s_%s" % (est, grid)
+            Wm = W if mode == 2 else None
+            # snapshot 0 on the full grid at R and 4R: cost model, rays/s, results
+            a, b, ts, st0, res = solve_cost(wl, R, fields, mode=mode, mlp=Wm, mix_eps=args.mix_eps, **kw)
+            e = study.errors(res["Q"][cells], Qref, vref)
+            leg = dict(rms_at_bench_R=e["rms"], max_at_bench_R=e["max"],
+                       rms_from_own_variance=math.sqrt(float(np.sum(res["sdev"][cells] ** 2)) /
+                                                       float(np.sum(Qref ** 2))),
+                       rays_per_s_snapshot0=st0["n_rays"] / ts[0],
+                       segments_per_ray=st0["n_steps"] / max(st0["n_rays"], 1),
+                       solve_s_snapshot0={str(R): ts[0], str(4 * R): ts[1]},
+                       cost_model_s=dict(fixed=a, per_ray_per_cell=b))
+            if mode == 2:
+                leg["proposal"] = info
+                leg["variance_ratio_vs_plain"] = float(np.sum(legs["plain_" + grid]["_sd2"]) / np.sum(res["sdev"] ** 2))
+            leg["_sd2"] = res["sdev"] ** 2
+            # rays per cell to 1% RMS, measured by doubling R on the sampled cells
+            tt = study.rays_to_target(wl, cells, Qref, vref, fields, mode=mode, mlp=Wm, mix_eps=args.mix_eps,
+                                      R0=R, Rmax=args.rmax, **kw)
+            leg["to_1pct"] = tt
+            R1 = tt["rays_per_cell"]
+            leg["time_to_1pct_s"] = cfg.nsnap * (a + b * R1) if R1 else None
+            R1f = tt["fit"]["rays_per_cell"]
+            leg["time_to_1pct_s_fit"] = cfg.nsnap * (a + b * R1f) if R1f else None
+            legs[name] = leg
+            log(name, {k: v for k, v in leg.items() if k not in ("_sd2", "to_1pct", "proposal")})
+    for leg in legs.values():
+        leg.pop("_sd2", None)
+
+    def ratio(x, y):
+        return x / y if (x and y) else None
+    out = dict(reference=dict(rays_per_cell=args.ref_rays, cells=len(cells), seconds=round(t_ref, 1),
+                              rms_noise=ref_noise, grid="fine (grid_num 1)", snapshot=0,
+                              kind="converged FP64 MC (FP64 traversal and tallies, other seed), the oracle's "
+                                   "estimator; its own variance is removed from every error"),
+               legs=legs)
     out["time_to_1pct_rms_s"] = dict(
-        plain=T1_p, learned=T1_l, speedup_learned=T1_p / T1_l,
-        rays_per_cell_to_1pct=dict(plain=R1_p, learned=R1_l),
-        solve_s_snapshot0=dict(plain={str(R): lin["plain"][2][0], str(4 * R): lin["plain"][2][1]},
-                               learned={str(R): lin["learned"][2][0], str(4 * R): lin["learned"][2][1]}),
-        at_bench_rays=dict(plain=t_p, learned=t_l, speedup_learned=t_p / t_l),
-        note="rays per cell to 1%% RMS from the error at %d rays/cell (error ~ R^-1/2); solve time at that "
-             "R from the measured fixed + per-ray cost on snapshot 0 (R and 4R, median of three solves each) times %d snapshots; "
-             "at_bench_rays scales the whole measured solve linearly instead" % (R, ns))
+        plain=legs["plain_fine"]["time_to_1pct_s"], learned=legs["learned_fine"]["time_to_1pct_s"],
+        speedup_learned=ratio(legs["plain_fine"]["time_to_1pct_s"], legs["learned_fine"]["time_to_1pct_s"]),
+        plain_amc=legs["plain_amc"]["time_to_1pct_s"], learned_amc=legs["learned_amc"]["time_to_1pct_s"],
+        speedup_learned_amc=ratio(legs["plain_amc"]["time_to_1pct_s"], legs["learned_amc"]["time_to_1pct_s"]),
+        amc_bias_rms=dict(plain=legs["plain_amc"]["to_1pct"]["fit"]["bias_rms"],
+                          learned=legs["learned_amc"]["to_1pct"]["fit"]["bias_rms"]),
+        note="rays per cell to 1%% RMS measured by doubling R from %d on %d cells of snapshot 0 against the "
+             "fine-grid FP64 reference; time = %d snapshots x (fixed + per-ray cost of a full-grid solve, "
+             "measured at R and 4R)" % (R, len(cells), cfg.nsnap))
+    return out
+
+
+def other_configs(args):
+    """The other configurations of the workload list, one key each: rays/s of
+    the full solve at the configuration's rays per cell and its RMS / max error
+    against a converged FP64 reference on 1024 sampled cells (grey 32^3 also
+    isothermal against the plane-parallel E2 solution); the 256x128x128
+    precision study (FP32 / FP64 traversal x FP64 / Kahan-FP32 tallies)."""
+    from dnsmc_amd import study, workloads
+    out = {}
+    for key, wl in (("grey32", "grey32"), ("ck64", "ck64"), ("ck256", "ck256")):
+        w = workloads.WORKLOADS[wl]
+        R = w["rays"]
+        fields = workloads.workload_fields(wl, 1)
+        cells = study.sample_cells(wl, args.ref_cells)
+        Qref, vref, t_ref = study.reference(wl, cells, args.cfg_ref_rays, fields=fields)
+        S, cfg = study.make_solver(wl, R, fields=fields)
+        S.run()
+        S.wait()
+        t0 = time.perf_counter()
+        S.run()
+        st = S.stats()
+        dt = time.perf_counter() - t0
+        r = S.results(0)
+        S.close()
+        e = study.errors(r["Q"][cells], Qref, vref)
+        d = dict(grid=list(cfg.n), rays_per_cell=R, rays_per_s=st["n_rays"] / dt, segments_per_s=st["n_steps"] / dt,
+                 solve_s=dt, rms=e["rms"], max=e["max"],
+                 reference=dict(rays_per_cell=args.cfg_ref_rays, cells=len(cells), seconds=round(t_ref, 2),
+                                rms_noise=math.sqrt(float(np.sum(vref)) / float(np.sum(Qref ** 2)))))
+        if wl == "grey32":
+            d["isothermal_vs_E2"] = grey_isothermal(R)
+        if wl == "ck256":
+            d["precision"] = precision_variants(wl, R, fields, cells, Qref, vref)
+        out[key] = d
+        log(key, {k: v for k, v in d.items() if k != "precision"})
+    return out
+
+
+def grey_isothermal(R):
+    """32^3 grey channel at exactly 1000 K, walls 300 K, kappa = 1: Q(y) =
+    2 kappa sigma (T^4 - Tw^4) (E2(kappa y) + E2(kappa (L - y))), the exact
+    expectation of the cell-centre estimator but for the transmissivity
+    cut-off (toll 1e-3), which moves it by less than 1e-3 relative."""
+    from scipy.special import expn
+    from dnsmc_amd import study, workloads
+    n = workloads.WORKLOADS["grey32"]["n"]
+    S, cfg = study.make_solver("grey32", R, fields=[np.full(n, 1000.0, np.float32)])
+    S.run()
+    r = S.results(0)
+    fy = S.array("faces1", 0).astype(np.float64)
+    S.close()
+    y = 0.5 * (fy[1:] + fy[:-1])
+    L = fy[-1]
+    want = 2.0 * 5.670373e-08 * (1000.0 ** 4 - 300.0 ** 4) * (expn(2, y) + expn(2, L - y))
+    Q = r["Q"].reshape(n[2], n[1], n[0])
+    diff = Q - want[None, :, None]
+    return dict(rms=float(np.sqrt(np.sum(diff ** 2) / (np.sum(want ** 2) * n[0] * n[2]))),
+                max=float(np.max(np.abs(diff)) / np.max(np.abs(want))),
+                plane_mean_max_rel=float(np.max(np.abs(Q.mean(axis=(0, 2)) - want) / np.abs(want))))
+
+
+def precision_variants(wl, R, fields, cells, Qref, vref):
+    """The same Philox rays four ways: against the FP64-traversal / FP64-tally
+    run (arithmetic error alone, all cells) and the converged reference."""
+    from dnsmc_amd import study
+    runs = {}
+    for prec, tr in ((0, "fp32"), (1, "fp64")):
+        for tally, tn in ((1, "fp64"), (2, "kahan_fp32")):
+            S, _ = study.make_solver(wl, R, precision=prec, tally=tally, fields=fields)
+            S.run()
+            S.wait()
+            t0 = time.perf_counter()
+            S.run()
+            S.wait()
+            dt = time.perf_counter() - t0
+            runs[(tr, tn)] = (S.results(0), dt)
+            S.close()
+    base = runs[("fp64", "fp64")][0]
+    out = {}
+    for (tr, tn), (r, dt) in runs.items():
+        same = study.errors(r["Q"], base["Q"])
+        conv = study.errors(r["Q"][cells], Qref, vref)
+        out["%s_traversal_%s_tally" % (tr, tn)] = dict(same_rays_rms=same["rms"], same_rays_max=same["max"],
+                                                       converged_rms=conv["rms"], converged_max=conv["max"],
+                                                       solve_s=dt)
     return out
 
 
@@ -220,16 +360,18 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=float, default=None, help="rays per traversal launch")
-    ap.add_argument("--copy-gib", type=float, default=1.0, help="size of the achievable-bandwidth copy (0: skip)")
+    ap.add_argument("--copy-gib", type=float, default=8.0, help="GiB of the achievable-bandwidth streams (0: skip)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg and the error study")
-    ap.add_argument("--no-amc", action="store_true", help="skip the leg with the reference's default mesh coarsening")
+    ap.add_argument("--no-configs", action="store_true", help="skip the grey 32^3, 64x96x64 and 256x128x128 keys")
+    ap.add_argument("--rmax", type=int, default=1 << 23, help="largest rays per cell of the doubling study")
     ap.add_argument("--train-snaps", type=int, default=20)
     ap.add_argument("--train-rays", type=int, default=8)
     ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--cost0", type=float, default=16.0)
     ap.add_argument("--mix-eps", type=float, default=0.1)
     ap.add_argument("--ref-cells", type=int, default=1024)
-    ap.add_argument("--ref-rays", type=int, default=65536)
+    ap.add_argument("--ref-rays", type=int, default=6000000)
+    ap.add_argument("--cfg-ref-rays", type=int, default=65536)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -282,8 +424,10 @@ def main():
         "config": {"workload": args.workload, "grid": list(cfg.n), "snapshots": cfg.nsnap, "rays_per_cell": R,
                    "table": "grey kappa=1" if table is None else "synthetic c-k 100x16x69",
                    "estimator": "plain (Tmax proposal, Philox)", "tally": "fp64", "tables": tables,
-                   "coarsening": "none (grid_num 1); reference_amc repeats the comparison with the reference's "
-                                 "default 5-level coarsening"},
+                   "coarsening": "none (grid_num 1); accuracy.legs repeat the comparison with the reference's "
+                                 "default 5-level coarsening",
+                   "timed_region": "fields already in HBM: dnsmc_set_field (host copy, bricking, coarse levels, "
+                                   "about 4 ms per 384x256x256 snapshot) runs before it"},
         "segments_per_s": nsteps / dt,
         "segments_per_ray": nsteps / max(nrays, 1),
         "ms_trace_per_step": ms_trace / args.steps,
@@ -310,35 +454,35 @@ def main():
         import ctypes
         from dnsmc_amd._lib import check, lib
         g = ctypes.c_double(0.0)
-        check(lib().dnsmc_bench_copy(int(args.copy_gib * 2 ** 30), 5, ctypes.byref(g)))
-        out["roofline"]["achievable_copy_GBs"] = g.value
-        out["roofline"]["frac_of_achievable"] = achieved / g.value
+        check(lib().dnsmc_bench_copy(int(2 ** 30), 5, ctypes.byref(g)))
+        out["roofline"]["copy_f32x4_1GiB_GBs"] = g.value   # the byte counters' calibration kernel
+        nb = int(args.copy_gib * 2 ** 30)
+        best = ctypes.c_int(-1)
+        check(lib().dnsmc_bench_stream(nb, 5, 1, ctypes.byref(g), ctypes.byref(best)))
+        rd = g.value
+        check(lib().dnsmc_bench_stream(nb, 5, 0, ctypes.byref(g), ctypes.byref(best)))
+        cp = g.value
+        r = out["roofline"]
+        r["achievable_read_GBs"] = rd
+        r["achievable_copy_GBs"] = cp
+        r["achievable_note"] = ("best of 12 streaming variants over %.0f GiB (16 B per lane, 1 or 4 per iteration, "
+                                "plain or non-temporal, 4 to 32 blocks per CU); the traversal reads "
+                                "(writes are 0.5 %% of its traffic), so the read-only rate is its ceiling"
+                                % args.copy_gib)
+        r["frac_of_achievable"] = achieved / rd
         if prof:
-            out["roofline"]["traffic_frac_of_achievable_whole_device"] = out["roofline"]["traffic_GBs_whole_device"] / g.value
+            r["traffic_frac_of_achievable_whole_device"] = r["traffic_GBs_whole_device"] / rd
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(S, cfg, fields, R, args.cpu_seconds)
         except Exception as e:  # reported, never fatal to the GPU number
             log("cpu baseline failed: %r" % (e,))
+    S.close()
     if not args.no_learned:
-        res_p = S.results(0)
-        S.close()
-        out.update(learned_leg(args, cfg, R, fields, res_p, ms_step / 1000.0, {}))
-        if not args.no_amc:
-            # the same comparison with the reference's default adaptive mesh
-            # coarsening (code/param.h: 5 levels, 5 steps on each of the first four)
-            amc = dict(grid_num=5, maxs=(5, 5, 5, 5, 100000, 100000))
-            from dnsmc_amd import study
-            SA, _ = study.make_solver(args.workload, R, nsnap=cfg.nsnap, fields=fields, **amc)
-            dt_a, st_a = timed_solve(SA, 1)
-            res_a = SA.results(0)
-            SA.close()
-            leg = learned_leg(args, cfg, R, fields, res_a, dt_a, amc)
-            out["reference_amc"] = dict(
-                config="grid_num 5, maxs 5 5 5 5 100000 (the reference's code/param.h defaults), else as above",
-                plain=dict(rays_per_s=st_a[0]["n_rays"] / dt_a, ms_per_step=dt_a * 1000.0,
-                           segments_per_ray=st_a[0]["n_steps"] / max(st_a[0]["n_rays"], 1)),
-                **leg)
+        out["accuracy"] = accuracy_legs(args, cfg, R, fields)
+        out["time_to_1pct_rms_s"] = out["accuracy"].pop("time_to_1pct_rms_s")
+    if not args.no_configs:
+        out["configs"] = other_configs(args)
     print(json.dumps(out), flush=True)
 
 

@@ -88,6 +88,48 @@ def errors(Q, Qref, var_ref=None):
                 max=float(np.max(np.abs(Q - Qref)) / np.max(np.abs(Qref))))
 
 
+def rays_to_target(workload, cells, Qref, vref, fields, mode=0, mlp=None, mix_eps=0.1, target=0.01, R0=100,
+                   Rmax=1 << 23, seed=None, **cfg_kw):
+    """Rays per cell to reach `target` RMS error against (Qref, vref), measured
+    by doubling: the estimator is run on the emitters `cells` of snapshot 0
+    (their errors are those of the full-grid solve: every cell's estimate is
+    independent of which other cells emit) at R0, 2 R0, ... until the error
+    is at most `target` or R exceeds Rmax.  Every point's error has the
+    reference's own variance removed.  Returns the points, the first R that
+    reached the target with its error, and the least-squares fit of
+    eps^2 = b^2 + c / R over the points (b: the estimator's bias against the
+    reference, e.g. mesh coarsening; R_fit = c / (target^2 - b^2))."""
+    pts = []
+    R = R0
+    first = None
+    while R <= Rmax:
+        S, _ = make_solver(workload, R, mode=mode, mlp=mlp, mix_eps=mix_eps, cells=cells, fields=fields[:1],
+                           seed=seed, **cfg_kw)
+        t0 = time.time()
+        S.run()
+        r = S.results(0)
+        dt = time.time() - t0
+        S.close()
+        e = errors(r["Q"], Qref, vref)
+        pts.append(dict(rays_per_cell=R, rms=e["rms"], max=e["max"], seconds=round(dt, 3)))
+        log("rays_to_target", workload, "mode", mode, cfg_kw.get("grid_num", 1), pts[-1])
+        if e["rms"] <= target:
+            first = pts[-1]
+            break
+        R *= 2
+    # eps^2 = b^2 + c/R, least squares over the points with the noise-dominated
+    # ones weighted alike (relative residuals)
+    Rs = np.array([p["rays_per_cell"] for p in pts], float)
+    e2 = np.array([p["rms"] ** 2 for p in pts], float)
+    A = np.stack([np.ones_like(Rs), 1.0 / Rs], axis=1) / e2[:, None]
+    coef, *_ = np.linalg.lstsq(A, np.ones_like(Rs), rcond=None)
+    b2, c = float(max(coef[0], 0.0)), float(max(coef[1], 0.0))
+    R_fit = c / (target ** 2 - b2) if b2 < target ** 2 and c > 0 else None
+    return dict(points=pts, target=target, reached=first is not None,
+                rays_per_cell=first["rays_per_cell"] if first else None, rms_at_R=first["rms"] if first else None,
+                fit=dict(bias_rms=math.sqrt(b2), c=c, rays_per_cell=R_fit))
+
+
 def study_errors(workloads_=("grey32", "ck64", "ck256"), ncells=1024, rays_ref=65536):
     out = {}
     for wl in workloads_:
