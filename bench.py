@@ -233,8 +233,10 @@ def accuracy_legs(args, cfg, R, fields):
             tt = study.rays_to_target(wl, cells, Qref, vref, fields, mode=mode, mlp=Wm, mix_eps=args.mix_eps,
                                       R0=R, Rmax=args.rmax, **kw)
             leg["to_1pct"] = tt
-            R1 = tt["rays_per_cell"]
+            R1 = tt["rays_per_cell_interp"]
             leg["time_to_1pct_s"] = cfg.nsnap * (a + b * R1) if R1 else None
+            R2 = tt["rays_per_cell"]
+            leg["time_to_1pct_s_first_doubling"] = cfg.nsnap * (a + b * R2) if R2 else None
             R1f = tt["fit"]["rays_per_cell"]
             leg["time_to_1pct_s_fit"] = cfg.nsnap * (a + b * R1f) if R1f else None
             legs[name] = leg
@@ -254,10 +256,15 @@ def accuracy_legs(args, cfg, R, fields):
         speedup_learned=ratio(legs["plain_fine"]["time_to_1pct_s"], legs["learned_fine"]["time_to_1pct_s"]),
         plain_amc=legs["plain_amc"]["time_to_1pct_s"], learned_amc=legs["learned_amc"]["time_to_1pct_s"],
         speedup_learned_amc=ratio(legs["plain_amc"]["time_to_1pct_s"], legs["learned_amc"]["time_to_1pct_s"]),
+        speedup_learned_amc_vs_plain_fine=ratio(legs["plain_fine"]["time_to_1pct_s"],
+                                                legs["learned_amc"]["time_to_1pct_s"]),
+        rays_per_cell_to_1pct={k: v["to_1pct"]["rays_per_cell_interp"] for k, v in legs.items()},
         amc_bias_rms=dict(plain=legs["plain_amc"]["to_1pct"]["fit"]["bias_rms"],
                           learned=legs["learned_amc"]["to_1pct"]["fit"]["bias_rms"]),
         note="rays per cell to 1%% RMS measured by doubling R from %d on %d cells of snapshot 0 against the "
-             "fine-grid FP64 reference; time = %d snapshots x (fixed + per-ray cost of a full-grid solve, "
+             "fine-grid FP64 reference, interpolated between the two doublings that bracket 1%% (each leg's "
+             "to_1pct holds the points, the first R below 1%% with its error, and the fit eps^2 = b^2 + c/R whose b "
+             "is the coarsening bias); time = %d snapshots x (fixed + per-ray cost of a full-grid solve, "
              "measured at R and 4R)" % (R, len(cells), cfg.nsnap))
     return out
 

@@ -96,7 +96,8 @@ def rays_to_target(workload, cells, Qref, vref, fields, mode=0, mlp=None, mix_ep
     independent of which other cells emit) at R0, 2 R0, ... until the error
     is at most `target` or R exceeds Rmax.  Every point's error has the
     reference's own variance removed.  Returns the points, the first R that
-    reached the target with its error, and the least-squares fit of
+    reached the target with its error, the R at the target interpolated
+    between the two points that bracket it, and the least-squares fit of
     eps^2 = b^2 + c / R over the points (b: the estimator's bias against the
     reference, e.g. mesh coarsening; R_fit = c / (target^2 - b^2))."""
     pts = []
@@ -125,9 +126,18 @@ def rays_to_target(workload, cells, Qref, vref, fields, mode=0, mlp=None, mix_ep
     coef, *_ = np.linalg.lstsq(A, np.ones_like(Rs), rcond=None)
     b2, c = float(max(coef[0], 0.0)), float(max(coef[1], 0.0))
     R_fit = c / (target ** 2 - b2) if b2 < target ** 2 and c > 0 else None
+    # R at the target between the two measured points that bracket it
+    # (log-log interpolation: a local power law between neighbours)
+    R_int = None
+    if first is not None and len(pts) >= 2:
+        (Ra, ea), (Rb, eb) = [(p["rays_per_cell"], p["rms"]) for p in pts[-2:]]
+        if ea > target >= eb and ea > eb:
+            R_int = Ra * (Rb / Ra) ** (math.log(ea / target) / math.log(ea / eb))
+    elif first is not None:
+        R_int = float(first["rays_per_cell"])
     return dict(points=pts, target=target, reached=first is not None,
                 rays_per_cell=first["rays_per_cell"] if first else None, rms_at_R=first["rms"] if first else None,
-                fit=dict(bias_rms=math.sqrt(b2), c=c, rays_per_cell=R_fit))
+                rays_per_cell_interp=R_int, fit=dict(bias_rms=math.sqrt(b2), c=c, rays_per_cell=R_fit))
 
 
 def study_errors(workloads_=("grey32", "ck64", "ck256"), ncells=1024, rays_ref=65536):
