@@ -22,7 +22,8 @@ EXPORTED = [
     "dnsmc_load_tables", "dnsmc_probe_tables", "dnsmc_set_tables", "dnsmc_set_field", "dnsmc_set_mlp",
     "dnsmc_set_emitters_slab", "dnsmc_set_emitters_list", "dnsmc_run", "dnsmc_run_noalloc", "dnsmc_wait",
     "dnsmc_get_results", "dnsmc_get_results_fortran", "dnsmc_get_stats", "dnsmc_get_array",
-    "dnsmc_get_rays", "dnsmc_get_ray_records", "dnsmc_set_record", "dnsmc_sync", "dnsmc_bench_copy",
+    "dnsmc_get_rays", "dnsmc_get_ray_records", "dnsmc_set_record", "dnsmc_set_draw_log", "dnsmc_sync",
+    "dnsmc_bench_copy",
     "dnsmc_bench_stream",
     "dnsmc_mlp_nparams", "dnsmc_mlp_samples", "dnsmc_mlp_features", "dnsmc_mlp_grad", "dnsmc_mlp_adam",
     "dnsmc_mlp_forward", "dnsmc_dev_alloc", "dnsmc_dev_free", "dnsmc_dev_copy",
@@ -119,6 +120,7 @@ def lib():
     L.dnsmc_get_ray_records.argtypes = [vp, vp, vp, vp, vp, vp, C.c_longlong]
     L.dnsmc_get_ray_records.restype = C.c_longlong
     L.dnsmc_set_record.argtypes = [vp, i32]
+    L.dnsmc_set_draw_log.argtypes = [vp, i32]
     L.dnsmc_sync.argtypes = []
     L.dnsmc_bench_copy.argtypes = [C.c_longlong, i32, f64p]
     L.dnsmc_bench_stream.argtypes = [C.c_longlong, i32, i32, f64p, C.POINTER(C.c_int)]

@@ -10,6 +10,7 @@ namespace dn {
 
 constexpr int kLevWords = 16;   // per-level parameter words staged in LDS
 constexpr int kRing = 8;        // tally ring entries per wave (power of 2)
+constexpr int kLRowFloats = 2048;   // a learned proposal row in a wave's row buffer (4 chunks x 64 lanes x 8)
 
 // Per-level geometry, staged into LDS as ints (floats for nothing).
 struct LevParams {
@@ -107,6 +108,24 @@ struct TraceArgs {
     uint32_t* rec_hash;
     float* rec_D;
     float* rec_G;
+    // learned proposal drawn inside the traversal (GEN == 2): the network,
+    // the plain bin probabilities and CDF, the mixture weight, the features'
+    // inputs (level-0 temperatures of snapshot 0, feat_stride floats per
+    // snapshot); with log_rows set, every proposal row the waves evaluate
+    // (cumulative weights U, noutp floats per processing slot) and every
+    // drawn ray (descriptor words, p/q in log_w) are written out
+    MlpWeights mlpw;
+    const float* p_bg;
+    const float* pcdf;
+    const float* pdir;          // the direction head's plain CDF, (d + 1) / 64
+    float mix_eps;
+    FeatArgs feat;
+    long long feat_stride;
+    float* lrow_cache;          // per wave: the row buffer (kLRowFloats floats)
+    float* log_rows;
+    uint32_t* log_w0;
+    uint32_t* log_w1;
+    float* log_w;
 };
 
 struct GenArgs {
@@ -178,5 +197,83 @@ struct SplineArgs {
     int brick_out, m0, nbx, nby;   // z pass: write the bricked layout of the coarse level
     int in_bricked, fn1, fnbx, fnby;   // x pass: read the bricked fine field (line = j + n1 k)
 };
+
+// ------------------------------------------------ shared device helpers ----
+// FP32 -> BF16, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ uint16_t f2bf(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Features of 0-based cell (i, j, k): position along the wall-normal axis in
+// [-1, 1], T/Tmax, the temperature gradient (central differences, periodic
+// wrap or one-sided at walls) scaled by (L/2)/Tmax, mean wall T / Tmax.
+__device__ __forceinline__ void cell_features(const FeatArgs& f, int i, int j, int k, float* out)
+{
+    const int c[3] = {i, j, k};
+    const float T = f.T0[brick_index(i, j, k, f.nbx, f.nby)];
+    float g[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int n = f.n[ax];
+        const bool per = (f.bc_periodic >> (2 * ax)) & 1;
+        int cm = c[ax] - 1, cp = c[ax] + 1;      // neighbour cells; centres index +1 (ghost-inclusive)
+        int tm = cm, tp = cp;
+        if (per) {
+            tm = cm < 0 ? n - 1 : cm;
+            tp = cp >= n ? 0 : cp;
+        } else {
+            cm = tm = cm < 0 ? 0 : cm;
+            cp = tp = cp >= n ? n - 1 : cp;
+        }
+        if (n < 2) {
+            g[ax] = 0.f;
+            continue;
+        }
+        int im[3] = {i, j, k}, ip[3] = {i, j, k};
+        im[ax] = tm;
+        ip[ax] = tp;
+        const float Tm = f.T0[brick_index(im[0], im[1], im[2], f.nbx, f.nby)];
+        const float Tp = f.T0[brick_index(ip[0], ip[1], ip[2], f.nbx, f.nby)];
+        const float xm = f.centres[f.coff[ax] + cm + 1], xp = f.centres[f.coff[ax] + cp + 1];
+        g[ax] = (Tp - Tm) / (xp - xm) * (0.5f * f.Lpolar) * f.inv_Tmax;
+    }
+    const int cpol = f.polar == 0 ? i : (f.polar == 1 ? j : k);
+    const int opol = f.polar == 0 ? f.coff[0] : (f.polar == 1 ? f.coff[1] : f.coff[2]);
+    const float y = f.centres[opol + cpol + 1];
+    out[0] = 2.0f * y / f.Lpolar - 1.0f;
+    out[1] = T * f.inv_Tmax;
+    out[2] = g[0];
+    out[3] = g[1];
+    out[4] = g[2];
+    out[5] = f.Tw_rel;
+}
+
+
+// Reference-style bin search (code/mc_gpu.cu:842-853 semantics) on the
+// mixture CDF F(k) = (1 - eps) U[k] / U[n-1] + eps P[k], evaluated per probe;
+// returns the bin and its probability F(k) - F(k-1).
+__device__ __forceinline__ int mix_search(const float* Uh, const float* P, int n, float eps, float R, float* qk)
+{
+    const float s = (1.0f - eps) / Uh[n - 1];
+    auto F = [&](int k) { return k < 0 ? 0.f : (k == n - 1 ? 1.0f : fminf(fmaf(Uh[k], s, eps * P[k]), 1.0f)); };
+    int lo = 0, w = n;
+    while (w > lo + 1) {
+        const int d = (lo + w) >> 1;
+        const float fd = F(d);
+        if (R < fd) w = d;
+        if (R >= fd) lo = d;
+    }
+    int k = lo + 1;
+    if (R < F(0)) k = 0;
+    if (R == 1.0f) k = n - 1;
+    if (k > n - 1) k = n - 1;
+    *qk = F(k) - F(k - 1);
+    return k;
+}
 
 }  // namespace dn

@@ -447,6 +447,162 @@ __device__ __forceinline__ RayStart pull_ray(const RayStart& r, int src)
     return o;
 }
 
+// ------------------------------------------ learned draw inside the traversal
+// The learned proposal of one emitting cell is evaluated by the whole wave
+// whose queue draws that cell's rays: the network (features, two tanh
+// layers, the output layer) and the running sums U of exp(clamp(z, -60, 60))
+// over each head, written to the wave's row buffer in the layout of
+// mlp_proposal's rows ([64 direction bins | nbin (band, g) bins | padding]).
+// Each ray then searches the defensive mixture of both heads in that row
+// (mix_search, as raygen_learned does from its rows).  It replaces, for the
+// learned estimator, the draw from the fixed proposal that the reference
+// makes where a bundle starts (code/mc_gpu.cu:138-144, find_band :830-896).
+// Lane L computes columns col = 512 j + 8 L + c (j < 4, c < 8): the output
+// layer's weights are then read 1 KB-contiguous per load instruction.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// c + a.lo b.lo + a.hi b.hi on BF16 pairs, FP32 accumulation (v_dot2_f32_bf16).
+__device__ __forceinline__ float dot2bf(uint32_t a, uint32_t b, float c)
+{
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, a), __builtin_bit_cast(bf16x2, b), c, false);
+}
+
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+
+// The row of emitting cell ijk (i | j << 10 | k << 20) of snapshot
+// `snapshot` into U (all 64 lanes call it): H1 = tanh(X W1 + b1), H2 =
+// tanh(H1 W2 + b2), Z = H2 W3 + b3, H1 and H2 rounded to BF16 as the training
+// kernels round them, the products in FP32 from BF16 pairs; U per head from
+// 8-column sums within a lane and exclusive lane scans across (the direction
+// head is lanes 0..7 of chunk 0).
+// An out-of-line call: the registers it needs are saved around the call,
+// once per row, instead of pressing on the traversal loop's registers.
+// Arguments are plain values (the network's pointers and sizes, the feature
+// inputs), so the kernel's argument block stays in its constant segment.
+__device__ __noinline__ void learned_row(const uint16_t* W1, const float* b1, const uint16_t* W2, const float* b2,
+                                         const uint16_t* W3, const float* b3, int nbin, int noutp, const float* T0,
+                                         int n0, int n1, int n2, int nbx, int nby, const float* centres, int co0,
+                                         int co1, int co2, int bcp, int polar, float Lpolar, float inv_Tmax,
+                                         float Tw_rel, int ijk, int lane, float* U)
+{
+    FeatArgs f;
+    f.T0 = T0;
+    f.n[0] = n0; f.n[1] = n1; f.n[2] = n2;
+    f.nbx = nbx; f.nby = nby;
+    f.centres = centres;
+    f.coff[0] = co0; f.coff[1] = co1; f.coff[2] = co2;
+    f.bc_periodic = bcp;
+    f.polar = polar;
+    f.Lpolar = Lpolar;
+    f.inv_Tmax = inv_Tmax;
+    f.Tw_rel = Tw_rel;
+    float x[kMlpIn];
+    cell_features(f, ijk & 1023, (ijk >> 10) & 1023, (ijk >> 20) & 1023, x);
+    MlpWeights w;
+    w.W1 = W1; w.b1 = b1; w.W2 = W2; w.b2 = b2; w.W3 = W3; w.b3 = b3;
+    w.nbin = nbin; w.noutp = noutp;
+    float a1 = w.b1[lane];
+#pragma unroll
+    for (int i = 0; i < kMlpIn; ++i) a1 = fmaf(x[i], bf2f(w.W1[i * kMlpHid + lane]), a1);
+    const uint32_t h1 = f2bf(tanhf(a1));
+    float a2 = w.b2[lane];
+    for (int kp = 0; kp < kMlpHid / 2; ++kp) {
+        const uint32_t hp = readlane_u(h1, 2 * kp) | (readlane_u(h1, 2 * kp + 1) << 16);
+        const uint32_t wp = (uint32_t)w.W2[(2 * kp) * kMlpHid + lane] | ((uint32_t)w.W2[(2 * kp + 1) * kMlpHid + lane] << 16);
+        a2 = dot2bf(hp, wp, a2);
+    }
+    const uint32_t h2 = f2bf(tanhf(a2));
+    const int nout = kMlpDir + w.nbin;
+    float base = 0.f;   // the (band, g) head's sum over the previous chunks
+    for (int j = 0; j < 4; ++j) {
+        const int col = 512 * j + 8 * lane;
+        float a[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) a[c] = 0.f;
+        if (col < noutp) {
+            const uint16_t* r = w.W3 + col;
+#pragma unroll 2
+            for (int kp = 0; kp < kMlpHid / 2; ++kp) {
+                const uint32_t hp = readlane_u(h2, 2 * kp) | (readlane_u(h2, 2 * kp + 1) << 16);
+                const uint4 u0 = *(const uint4*)(r + (size_t)(2 * kp) * noutp);
+                const uint4 u1 = *(const uint4*)(r + (size_t)(2 * kp + 1) * noutp);
+                a[0] = dot2bf(hp, __builtin_amdgcn_perm(u1.x, u0.x, 0x05040100u), a[0]);
+                a[1] = dot2bf(hp, __builtin_amdgcn_perm(u1.x, u0.x, 0x07060302u), a[1]);
+                a[2] = dot2bf(hp, __builtin_amdgcn_perm(u1.y, u0.y, 0x05040100u), a[2]);
+                a[3] = dot2bf(hp, __builtin_amdgcn_perm(u1.y, u0.y, 0x07060302u), a[3]);
+                a[4] = dot2bf(hp, __builtin_amdgcn_perm(u1.z, u0.z, 0x05040100u), a[4]);
+                a[5] = dot2bf(hp, __builtin_amdgcn_perm(u1.z, u0.z, 0x07060302u), a[5]);
+                a[6] = dot2bf(hp, __builtin_amdgcn_perm(u1.w, u0.w, 0x05040100u), a[6]);
+                a[7] = dot2bf(hp, __builtin_amdgcn_perm(u1.w, u0.w, 0x07060302u), a[7]);
+            }
+        }
+        // e = exp(clamp(z)), running sums within the lane's 8 columns
+        float run = 0.f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (col + c < nout) run += __expf(fminf(fmaxf(a[c] + w.b3[col + c], -60.f), 60.f));
+            a[c] = run;
+        }
+        // exclusive scan of the lanes' sums, the direction head on its own
+        const int seg = (j == 0 && lane >= 8) ? 8 : 0;
+        float incl = run;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const float t = __shfl_up(incl, o);
+            if (lane - o >= seg) incl += t;
+        }
+        float excl = __shfl_up(incl, 1);
+        if (lane == seg) excl = 0.f;
+        const float sb = (j == 0 && lane < 8) ? excl : base + excl;
+        if (col < noutp) {
+            float4* d = (float4*)(U + col);
+            d[0] = make_float4(sb + a[0], sb + a[1], sb + a[2], sb + a[3]);
+            d[1] = make_float4(sb + a[4], sb + a[5], sb + a[6], sb + a[7]);
+        }
+        base = base + __shfl(incl, 63);
+    }
+}
+
+// mix_search on both heads of a row at once: the two binary searches advance
+// in the same loop (their probes are independent loads), each with exactly
+// mix_search's probe sequence and special cases, so the bins and
+// probabilities are mix_search's.
+__device__ __forceinline__ void mix_search2(const float* Ud, const float* Pd, int nd, const float* Ub,
+                                            const float* Pb, int nb, float eps, float Rd, float Rb, int* kd,
+                                            float* qd, int* kb, float* qb)
+{
+    const float sd = (1.0f - eps) / Ud[nd - 1], sb = (1.0f - eps) / Ub[nb - 1];
+    auto Fd = [&](int k) { return k < 0 ? 0.f : (k == nd - 1 ? 1.0f : fminf(fmaf(Ud[k], sd, eps * Pd[k]), 1.0f)); };
+    auto Fb = [&](int k) { return k < 0 ? 0.f : (k == nb - 1 ? 1.0f : fminf(fmaf(Ub[k], sb, eps * Pb[k]), 1.0f)); };
+    int lod = 0, wd = nd, lob = 0, wb = nb;
+    while (wd > lod + 1 || wb > lob + 1) {
+        const bool ad = wd > lod + 1, ab = wb > lob + 1;
+        const int dd = (lod + wd) >> 1, db = (lob + wb) >> 1;
+        const float fd = ad ? Fd(dd) : 0.f;
+        const float fb = ab ? Fb(db) : 0.f;
+        if (ad) {
+            if (Rd < fd) wd = dd;
+            if (Rd >= fd) lod = dd;
+        }
+        if (ab) {
+            if (Rb < fb) wb = db;
+            if (Rb >= fb) lob = db;
+        }
+    }
+    int k = lod + 1;
+    if (Rd < Fd(0)) k = 0;
+    if (Rd == 1.0f) k = nd - 1;
+    if (k > nd - 1) k = nd - 1;
+    *qd = Fd(k) - Fd(k - 1);
+    *kd = k;
+    k = lob + 1;
+    if (Rb < Fb(0)) k = 0;
+    if (Rb == 1.0f) k = nb - 1;
+    if (k > nb - 1) k = nb - 1;
+    *qb = Fb(k) - Fb(k - 1);
+    *kb = k;
+}
+
 // Persistent cell-by-cell traversal (code/mc_gpu.cu:228-438).
 //
 // Work: the launch's rays (in 3-D brick order of their emitting cells) are
@@ -482,12 +638,13 @@ __device__ __forceinline__ RayStart pull_ray(const RayStart& r, int src)
 // optical depth / transmissivity / tally update consumes kappa last.  With
 // the kappa/Ib slice in LDS the field holds table coordinates (t_to_u).
 //
-// GEN: the rays are drawn in the kernel (the plain estimator's Philox draw,
-// bit-identical to raygen_plain) instead of read from descriptors, and one
-// launch covers every snapshot: XCD x's claims walk snapshot by snapshot
-// through its 1/8 of each snapshot's super-chunks, so all XCDs work on the
-// same snapshot at a time.
-template <int SPEC, int TAB, int TMODE, bool REC, bool AMC, bool GEN>
+// GEN: the rays are drawn in the kernel instead of read from descriptors
+// (1: the plain estimator's Philox draw, bit-identical to raygen_plain; 2:
+// the learned proposal, evaluated per emitting cell by the wave that draws
+// its rays, see learned_row), and one launch covers every snapshot: XCD x's
+// claims walk snapshot by snapshot through its 1/8 of each snapshot's
+// super-chunks, so all XCDs work on the same snapshot at a time.
+template <int SPEC, int TAB, int TMODE, bool REC, bool AMC, int GEN>
 __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArgs P)
 {
     extern __shared__ float lds[];
@@ -591,6 +748,14 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     int n0 = s_lev[0], n1 = s_lev[1], n2 = s_lev[2];
     int fo0 = s_lev[3], fo1 = s_lev[4], fo2 = s_lev[5], maxs = s_lev[6];
     int oo0 = s_lev[11], oo1 = s_lev[12], oo2 = s_lev[13];
+    if (!AMC) {   // one level: its parameters are the same for every lane (scalar registers)
+        n0 = __builtin_amdgcn_readfirstlane(n0); n1 = __builtin_amdgcn_readfirstlane(n1);
+        n2 = __builtin_amdgcn_readfirstlane(n2);
+        fo0 = __builtin_amdgcn_readfirstlane(fo0); fo1 = __builtin_amdgcn_readfirstlane(fo1);
+        fo2 = __builtin_amdgcn_readfirstlane(fo2); maxs = __builtin_amdgcn_readfirstlane(maxs);
+        oo0 = __builtin_amdgcn_readfirstlane(oo0); oo1 = __builtin_amdgcn_readfirstlane(oo1);
+        oo2 = __builtin_amdgcn_readfirstlane(oo2);
+    }
     gfloat* Tl = lev_T(s_lev);
     gfloat* T0 = Tl;
     uint32_t rsnap = 0;
@@ -623,6 +788,14 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         unsafeAtomicAdd(&P.Q2[t], s_rv[3 * e + 2]);
     };
     RayStart Qr = {};
+    // learned draw (GEN == 2): this lane's drawn bins and p/q for draw_ray, and
+    // the wave's cached proposal row (processing slot and snapshot)
+    int lr_band = 0, lr_g = 0;
+    uint32_t lr_mt = 0, lr_mp = 0;
+    float lr_ratio = 0.f;
+    uint32_t row_slot = 0xFFFFFFFFu, row_snap = 0xFFFFFFFFu;
+    const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    float* const row_buf = GEN == 2 ? P.lrow_cache + (size_t)wave_id * (size_t)kLRowFloats : nullptr;
 
     // ray q of snapshot qs of the launch: drawn here (GEN, the plain estimator's
     // Philox draw, bit-identical to raygen_plain) or read from its descriptors
@@ -644,7 +817,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         uint32_t mt, mp;
         float ratio = 1.0f;       // p/q of a learned descriptor
         bool weigh = GEN;         // emission weight computed here (tables in LDS)
-        if (GEN) {
+        if (GEN == 1) {
             const uint32_t r = q - slot * (uint32_t)P.rays_per_slot;
             const u32x4 x = philox4x32_10(u32x4{r, (uint32_t)si.y, (uint32_t)(P.snap + (int)qs), 0u},
                                           (uint32_t)P.seed, (uint32_t)(P.seed >> 32));
@@ -658,6 +831,12 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
             g = g > P.nQ - 1 ? P.nQ - 1 : g;
             mt = x.z >> 8;
             mp = x.w >> 8;
+        } else if (GEN == 2) {
+            bnd = lr_band;
+            g = lr_g;
+            mt = lr_mt;
+            mp = lr_mp;
+            ratio = lr_ratio;
         } else {
             const uint32_t a0 = P.w0[q], a1 = P.w1[q];
             bnd = desc_band(a0, a1);
@@ -705,8 +884,8 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
             const float iem = (P.gen_lds & 2) ? s_iem[bnd * P.nQ + g] : P.inv_emax[bnd * P.nQ + g];
             const float w = P.Ibmax * (kap * ib) * iem;
             const float wi = w * (1.0f / ib);
-            rd.rw = GEN ? w : w * ratio;
-            rd.rwi = GEN ? wi : wi * ratio;
+            rd.rw = GEN == 1 ? w : w * ratio;
+            rd.rwi = GEN == 1 ? wi : wi * ratio;
         }
         const float ut = ((float)mt + P.u_off) * kTwoM24;
         const float up = ((float)mp + P.u_off) * kTwoM24;
@@ -840,6 +1019,63 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                     if (cur < cend) {
                         gn = min(64u, cend - cur);
                         gsnap = csnap;
+                        if (GEN == 2) {
+                            // the learned draws of rays cur .. cur + gn - 1: for each
+                            // processing slot among them (wave-uniform) its proposal
+                            // row in the wave's row buffer (evaluated unless it is the
+                            // row the buffer holds), then the rays of that slot search it
+                            const uint32_t q = cur + min((uint32_t)lane, gn - 1u);
+                            const uint32_t slot = fdiv(q, P.fd_R);
+                            const uint32_t r = q - slot * (uint32_t)P.rays_per_slot;
+                            const int4 si = P.slotinfo[P.slot0 + slot];
+                            const uint32_t snapshot = (uint32_t)P.snap + gsnap;
+                            const u32x4 x = philox4x32_10(u32x4{r, (uint32_t)si.y, snapshot, 1u}, (uint32_t)P.seed,
+                                                          (uint32_t)(P.seed >> 32));
+                            const uint32_t s0 = readlane_u(slot, 0), s1 = readlane_u(slot, (int)gn - 1);
+                            lr_band = 0; lr_g = 0; lr_mt = 0; lr_mp = 0; lr_ratio = 0.f;
+                            for (uint32_t sl = s0; sl <= s1; ++sl) {
+                                if (sl != row_slot || gsnap != row_snap) {
+                                    const FeatArgs& fa = P.feat;
+                                    learned_row(P.mlpw.W1, P.mlpw.b1, P.mlpw.W2, P.mlpw.b2, P.mlpw.W3, P.mlpw.b3,
+                                                P.mlpw.nbin, P.mlpw.noutp,
+                                                fa.T0 + (size_t)snapshot * (size_t)P.feat_stride, fa.n[0], fa.n[1],
+                                                fa.n[2], fa.nbx, fa.nby, fa.centres, fa.coff[0], fa.coff[1],
+                                                fa.coff[2], fa.bc_periodic, fa.polar, fa.Lpolar, fa.inv_Tmax,
+                                                fa.Tw_rel, P.slotinfo[P.slot0 + sl].w, lane, row_buf);
+                                    row_slot = __builtin_amdgcn_readfirstlane(sl);
+                                    row_snap = __builtin_amdgcn_readfirstlane(gsnap);
+                                    // the row's columns were written by every lane of the wave
+                                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                                    if (P.log_rows) {
+                                        const int no = P.mlpw.noutp;
+                                        float* lr = P.log_rows + (size_t)(P.slot0 + sl) * (size_t)no;
+                                        for (int cc = lane; cc < no; cc += 64) lr[cc] = row_buf[cc];
+                                    }
+                                }
+                                if (slot == sl) {
+                                    float qd, qb;
+                                    int d, bg;
+                                    mix_search2(row_buf, P.pdir, kMlpDir, row_buf + kMlpDir, P.pcdf, P.mlpw.nbin,
+                                                P.mix_eps, u01_open(x.x), u01_open(x.y), &d, &qd, &bg, &qb);
+                                    lr_band = bg / P.nQ;
+                                    lr_g = bg - lr_band * P.nQ;
+                                    lr_ratio = (qb > 0.f && qd > 0.f) ? (P.p_bg[bg] / qb) * ((1.0f / kMlpDir) / qd) : 0.f;
+                                    // direction bin d = 8 cos(theta) bins x 8 phi bins: the bin in
+                                    // the top 3 of the 24 uniform bits, the Philox bits below
+                                    lr_mt = ((uint32_t)(d >> 3) << 21) | (x.z >> 11);
+                                    lr_mp = ((uint32_t)(d & 7) << 21) | (x.w >> 11);
+                                }
+                            }
+                            if (P.log_w0 && (uint32_t)lane < gn) {
+                                uint32_t w0, w1;
+                                desc_pack(lr_mt, lr_mp, lr_band, lr_g, &w0, &w1);
+                                const size_t o = (size_t)gsnap * (size_t)P.nrays + (cur + (uint32_t)lane);
+                                P.log_w0[o] = w0;
+                                P.log_w1[o] = w1;
+                                P.log_w[o] = lr_ratio;
+                            }
+                        }
                         if ((uint32_t)lane < gn) Qr = draw_ray(cur + (uint32_t)lane, gsnap);
                         cur += gn;
                         // ring entries for the batch's processing slots (consecutive:
@@ -1275,8 +1511,8 @@ __global__ void reduce_rays64(const double2* pray, int tally, long long nslots, 
 
 #define DN_TRACE_INST(S, I, T, R, A, G) template __global__ void trace<S, I, T, R, A, G>(TraceArgs);
 #define DN_TRACE_INST2(S, I, T, R, G) DN_TRACE_INST(S, I, T, R, false, G) DN_TRACE_INST(S, I, T, R, true, G)
-#define DN_TRACE_INST3(S, I) DN_TRACE_INST2(S, I, 0, false, false) DN_TRACE_INST2(S, I, 0, true, false) \
-    DN_TRACE_INST2(S, I, 1, false, false) DN_TRACE_INST2(S, I, 0, false, true)
+#define DN_TRACE_INST3(S, I) DN_TRACE_INST2(S, I, 0, false, 0) DN_TRACE_INST2(S, I, 0, true, 0) \
+    DN_TRACE_INST2(S, I, 1, false, 0) DN_TRACE_INST2(S, I, 0, false, 1) DN_TRACE_INST2(S, I, 0, false, 2)
 DN_TRACE_INST3(kSpecTable, kTabSliced)
 DN_TRACE_INST3(kSpecTable, kTabIbLds)
 DN_TRACE_INST3(kSpecTable, kTabGlobal)

@@ -24,15 +24,6 @@ namespace dn {
 typedef short s4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16, round to nearest even (NaN kept quiet).
-__device__ __forceinline__ uint16_t f2bf(float x)
-{
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // Fragment of a 16-row panel starting at row r0, k-block k0: element (r, k)
 // of the operand is M[r * rs + k * cs].
@@ -409,50 +400,6 @@ __global__ __launch_bounds__(256) void mlp_forward_rows(const float* X, long lon
 }
 
 // ------------------------------------------------------------ features ----
-// Features of 0-based cell (i, j, k): position along the wall-normal axis in
-// [-1, 1], T/Tmax, the temperature gradient (central differences, periodic
-// wrap or one-sided at walls) scaled by (L/2)/Tmax, mean wall T / Tmax.
-__device__ __forceinline__ void cell_features(const FeatArgs& f, int i, int j, int k, float* out)
-{
-    const int c[3] = {i, j, k};
-    const float T = f.T0[brick_index(i, j, k, f.nbx, f.nby)];
-    float g[3];
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-        const int n = f.n[ax];
-        const bool per = (f.bc_periodic >> (2 * ax)) & 1;
-        int cm = c[ax] - 1, cp = c[ax] + 1;      // neighbour cells; centres index +1 (ghost-inclusive)
-        int tm = cm, tp = cp;
-        if (per) {
-            tm = cm < 0 ? n - 1 : cm;
-            tp = cp >= n ? 0 : cp;
-        } else {
-            cm = tm = cm < 0 ? 0 : cm;
-            cp = tp = cp >= n ? n - 1 : cp;
-        }
-        if (n < 2) {
-            g[ax] = 0.f;
-            continue;
-        }
-        int im[3] = {i, j, k}, ip[3] = {i, j, k};
-        im[ax] = tm;
-        ip[ax] = tp;
-        const float Tm = f.T0[brick_index(im[0], im[1], im[2], f.nbx, f.nby)];
-        const float Tp = f.T0[brick_index(ip[0], ip[1], ip[2], f.nbx, f.nby)];
-        const float xm = f.centres[f.coff[ax] + cm + 1], xp = f.centres[f.coff[ax] + cp + 1];
-        g[ax] = (Tp - Tm) / (xp - xm) * (0.5f * f.Lpolar) * f.inv_Tmax;
-    }
-    const int cpol = f.polar == 0 ? i : (f.polar == 1 ? j : k);
-    const int opol = f.polar == 0 ? f.coff[0] : (f.polar == 1 ? f.coff[1] : f.coff[2]);
-    const float y = f.centres[opol + cpol + 1];
-    out[0] = 2.0f * y / f.Lpolar - 1.0f;
-    out[1] = T * f.inv_Tmax;
-    out[2] = g[0];
-    out[3] = g[1];
-    out[4] = g[2];
-    out[5] = f.Tw_rel;
-}
-
 // Direction bin of descriptor words: 8 equal-area cos(theta) bins x 8 phi bins.
 __device__ __forceinline__ int desc_dir_bin(uint32_t w0, uint32_t w1)
 {
@@ -577,28 +524,6 @@ __global__ __launch_bounds__(256) void mlp_proposal(GenArgs a, float* U)
             __syncthreads();
         }
     }
-}
-
-// Reference-style bin search (code/mc_gpu.cu:842-853 semantics) on the
-// mixture CDF F(k) = (1 - eps) U[k] / U[n-1] + eps P[k], evaluated per probe;
-// returns the bin and its probability F(k) - F(k-1).
-__device__ __forceinline__ int mix_search(const float* Uh, const float* P, int n, float eps, float R, float* qk)
-{
-    const float s = (1.0f - eps) / Uh[n - 1];
-    auto F = [&](int k) { return k < 0 ? 0.f : (k == n - 1 ? 1.0f : fminf(fmaf(Uh[k], s, eps * P[k]), 1.0f)); };
-    int lo = 0, w = n;
-    while (w > lo + 1) {
-        const int d = (lo + w) >> 1;
-        const float fd = F(d);
-        if (R < fd) w = d;
-        if (R >= fd) lo = d;
-    }
-    int k = lo + 1;
-    if (R < F(0)) k = 0;
-    if (R == 1.0f) k = n - 1;
-    if (k > n - 1) k = n - 1;
-    *qk = F(k) - F(k - 1);
-    return k;
 }
 
 // raygen_learned: one block per 8 consecutive slots.  Their rows of U are

@@ -24,7 +24,7 @@
 
 namespace dn {
 // kernels.hip
-template <int SPEC, int TAB, int TMODE, bool REC, bool AMC, bool GEN>
+template <int SPEC, int TAB, int TMODE, bool REC, bool AMC, int GEN>
 __global__ void trace(TraceArgs P);
 __global__ void t_range(const float* T, int n0, int n1, int n2, int nbx, int nby, unsigned* lohi);
 __global__ void raygen_plain(GenArgs a);
@@ -236,6 +236,10 @@ struct dnsmc_solver {
     float mix_eps = 0.1f;
     DevBuf<uint16_t> d_W1b, d_W2b, d_W3b;
     DevBuf<float> d_b1, d_b2, d_b3, d_pbg, d_pcdf;
+    DevBuf<float> d_lcache;    // learned draw in the traversal: each wave's row buffer
+    DevBuf<float> d_pdir;      // the direction head's plain CDF
+    DevBuf<float> d_lrows;     // logged proposal rows of the last learned run (dnsmc_set_draw_log)
+    int draw_log = 0;
     // records
     int record = 0;
     long long nrec = 0;
@@ -596,7 +600,7 @@ static void build_slotinfo(const std::vector<int>& emit, const std::vector<int>&
 // device for this kernel variant (blocks per CU from the occupancy query,
 // times the CU count), capped by `max_blocks`, a multiple of 8 (one share per
 // XCD).  The sliced-table variant runs one 1024-thread block per CU.
-template <int SPEC, int TAB, int TM, bool REC, bool GEN = false>
+template <int SPEC, int TAB, int TM, bool REC, int GEN = 0>
 static hipError_t launch_trace_t(bool amc, int ncu, int max_blocks, size_t lds, hipStream_t st, const TraceArgs& P)
 {
     void (*k)(TraceArgs) = amc ? trace<SPEC, TAB, TM, REC, true, GEN> : trace<SPEC, TAB, TM, REC, false, GEN>;
@@ -614,17 +618,19 @@ static hipError_t launch_trace_t(bool amc, int ncu, int max_blocks, size_t lds, 
     return hipGetLastError();
 }
 
+// gen: 0 descriptors, 1 the plain Philox draw, 2 the learned draw, in the kernel
 template <int SPEC, int TAB>
-static hipError_t launch_trace_m(bool gen, bool det, bool rec, bool amc, int ncu, int max_blocks, size_t lds,
+static hipError_t launch_trace_m(int gen, bool det, bool rec, bool amc, int ncu, int max_blocks, size_t lds,
                                  hipStream_t st, const TraceArgs& P)
 {
-    if (gen) return launch_trace_t<SPEC, TAB, 0, false, true>(amc, ncu, max_blocks, lds, st, P);
+    if (gen == 2) return launch_trace_t<SPEC, TAB, 0, false, 2>(amc, ncu, max_blocks, lds, st, P);
+    if (gen) return launch_trace_t<SPEC, TAB, 0, false, 1>(amc, ncu, max_blocks, lds, st, P);
     if (rec) return launch_trace_t<SPEC, TAB, 0, true>(amc, ncu, max_blocks, lds, st, P);
     if (det) return launch_trace_t<SPEC, TAB, 1, false>(amc, ncu, max_blocks, lds, st, P);
     return launch_trace_t<SPEC, TAB, 0, false>(amc, ncu, max_blocks, lds, st, P);
 }
 
-static hipError_t launch_trace(int spec, int tab, bool gen, bool det, bool rec, bool amc, int ncu, int max_blocks,
+static hipError_t launch_trace(int spec, int tab, int gen, bool det, bool rec, bool amc, int ncu, int max_blocks,
                                size_t lds, hipStream_t st, const TraceArgs& P)
 {
     if (spec == kSpecGreyT4)
@@ -747,13 +753,21 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
     // consecutive slot ranges (ray indices within a launch are 32-bit)
     bool fused = c.mode == DNSMC_MODE_PLAIN && c.precision == 0 && c.tally == DNSMC_TALLY_F64_ATOMIC && !s->record;
     if (const char* fe = getenv("DNSMC_FUSED")) fused = fused && (atoi(fe) != 0 || na);
+    // learned estimator, atomic tallies, no records: the proposal is evaluated
+    // and the rays are drawn inside the traversal too, one launch for every
+    // snapshot (DNSMC_LEARNED_FUSED=0, or more (band, g) bins than the 1984 a
+    // wave's registers hold: the snapshot-wise generation below)
+    bool lfused = c.mode == DNSMC_MODE_LEARNED && s->have_mlp && c.precision == 0 &&
+                  c.tally == DNSMC_TALLY_F64_ATOMIC && !s->record && s->mlp_nbin <= 1984;
+    if (const char* le = getenv("DNSMC_LEARNED_FUSED")) lfused = lfused && atoi(le) != 0;
+    if (lfused) fused = true;
     if (fused) slots_per_batch = nslots;
     // learned estimator, atomic tallies, no records: the descriptors of a whole
     // snapshot are generated (batch by batch, on a second stream) before one
     // traversal launch covers them, and the next snapshot is generated while
     // this one is traced (two snapshot-sized descriptor sets; DNSMC_SNAPWISE=0:
     // batched)
-    bool snapwise = c.mode == DNSMC_MODE_LEARNED && s->have_mlp && c.precision == 0 &&
+    bool snapwise = !lfused && c.mode == DNSMC_MODE_LEARNED && s->have_mlp && c.precision == 0 &&
                     c.tally == DNSMC_TALLY_F64_ATOMIC && !s->record && nslots * (long long)R <= (1ll << 32) - (1ll << 20);
     if (const char* se = getenv("DNSMC_SNAPWISE")) snapwise = snapwise && atoi(se) != 0;
     if (snapwise) {
@@ -1038,6 +1052,36 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
         P.Q2 = tQ2;
         P.nrec = 0;
         P.claim = na ? claim0 : pp.d_claim.p;
+        if (lfused) {
+            P.mlpw = mlp_weights(s);
+            P.p_bg = s->d_pbg.p;
+            P.pcdf = s->d_pcdf.p;
+            {   // the direction head's plain CDF, as raygen_learned forms it
+                std::vector<float> pd(kMlpDir);
+                for (int e = 0; e < kMlpDir; ++e) pd[e] = (float)(e + 1) * (1.0f / kMlpDir);
+                PUT(s->d_pdir, pd);
+                P.pdir = s->d_pdir.p;
+            }
+            P.mix_eps = s->mix_eps;
+            P.feat = feat_args(s, d);
+            P.feat_stride = s->tstride[0];
+            // a row per resident wave (at most 32 waves per CU)
+            HIPCHK(s->d_lcache.ensure((size_t)s->ncu * 32 * kLRowFloats));
+            P.lrow_cache = s->d_lcache.p;
+            P.log_rows = nullptr;
+            P.log_w0 = nullptr; P.log_w1 = nullptr; P.log_w = nullptr;
+            if (s->draw_log) {
+                const long long nr = nslots * (long long)R * c.nsnap;
+                HIPCHK(s->d_lrows.ensure((size_t)nslots * mlp_noutp(s->mlp_nbin)));
+                HIPCHK(pp.d_w0.ensure(nr));
+                HIPCHK(pp.d_w1.ensure(nr));
+                HIPCHK(pp.d_w.ensure(nr));
+                HIPCHK(pp.d_winv.ensure(nr));   // not written: dnsmc_get_rays reads it
+                P.log_rows = s->d_lrows.p;
+                P.log_w0 = pp.d_w0.p; P.log_w1 = pp.d_w1.p; P.log_w = pp.d_w.p;
+                s->first_batch_rays = nslots * (long long)R;
+            }
+        }
         long long part = std::max<long long>(1, ((1ll << 32) - (1ll << 20)) / R);   // slots per launch
         if (const char* fp = getenv("DNSMC_FUSED_SLOTS")) part = std::max<long long>(1, std::min(part, atoll(fp)));
         for (long long p0 = 0; p0 < nslots; p0 += part) {
@@ -1049,7 +1093,7 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
             const int tb_cap = (int)std::min<long long>(trace_blocks, std::max<long long>(8, (nsc + 15) / 16));
             EventPair* et = na ? nullptr : &next_event(s, 1);
             if (et) HIPCHK(hipEventRecord(et->a, st));
-            HIPCHK(launch_trace(spec, tab, true, false, false, amc, s->ncu, tb_cap, lds_trace, st, P));
+            HIPCHK(launch_trace(spec, tab, lfused ? 2 : 1, false, false, amc, s->ncu, tb_cap, lds_trace, st, P));
             if (et) HIPCHK(hipEventRecord(et->b, st));
             s->st.n_trace_launches++;
         }
@@ -1748,6 +1792,19 @@ int dnsmc_get_array(dnsmc_solver* s, const char* name, int level, float* out, lo
         v.resize(b.size());
         for (long long ps = 0; ps < nsl; ++ps)
             memcpy(v.data() + (size_t)s->order[ps] * row, b.data() + (size_t)ps * row, (size_t)row * sizeof(float));
+    } else if (nm == "lrows") {
+        // the learned proposal rows the traversal evaluated in the last run
+        // with the draw log on (cumulative weights U, mlp_noutp(nbin) floats
+        // per slot), rows in emitter order
+        const long long nsl = (long long)s->emit.size() * spe_of(s->cfg);
+        const long long row = mlp_noutp(s->mlp_nbin);
+        if (!s->have_mlp || s->emit_dirty || (long long)s->d_lrows.n < nsl * row)
+            return fail(DNSMC_ESTATE, "no logged learned run covering every slot (dnsmc_set_draw_log)");
+        std::vector<float> b((size_t)(nsl * row));
+        HIPCHK(hipMemcpy(b.data(), s->d_lrows.p, b.size() * sizeof(float), hipMemcpyDeviceToHost));
+        v.resize(b.size());
+        for (long long ps = 0; ps < nsl; ++ps)
+            memcpy(v.data() + (size_t)s->order[ps] * row, b.data() + (size_t)ps * row, (size_t)row * sizeof(float));
     } else if (nm == "features") {
         // the learned proposal's input features of every slot of the last run
         // (kMlpIn floats per slot), rows in emitter order
@@ -1828,6 +1885,13 @@ long long dnsmc_get_ray_records(dnsmc_solver* s, int* steps, int* end_type, unsi
     if ((rc = fetch_perm<float>(s->d_rD.p, c, perm, D))) return rc;
     if ((rc = fetch_perm<float>(s->d_rG.p, c, perm, G))) return rc;
     return c;
+}
+
+int dnsmc_set_draw_log(dnsmc_solver* s, int on)
+{
+    if (!s) return fail(DNSMC_EINVAL, "null solver");
+    s->draw_log = on ? 1 : 0;
+    return DNSMC_OK;
 }
 
 int dnsmc_set_record(dnsmc_solver* s, int on)

@@ -165,6 +165,9 @@ class Solver:
              for i, x in enumerate((W1, b1, W2, b2, W3, b3))]
         check(lib().dnsmc_set_mlp(self.h, *[x.ctypes.data for x in a], C.c_float(mix_eps)))
 
+    def set_draw_log(self, on=True):
+        check(lib().dnsmc_set_draw_log(self.h, 1 if on else 0))
+
     def set_record(self, on=True):
         check(lib().dnsmc_set_record(self.h, 1 if on else 0))
 

@@ -269,7 +269,7 @@ int dnsmc_get_stats(dnsmc_solver *s, dnsmc_stats *st);
 
 /* Test and training hooks. */
 /* Derived arrays: name in {"faces0","faces1",..,"centres","T1".."T5","cdf_band","cdf_g",
- * "inv_emax","Ib","kq","Ibw","scalars","p_bg","pcdf","qcdf","features"}; copies up to n
+ * "inv_emax","Ib","kq","Ibw","scalars","p_bg","pcdf","qcdf","lrows","features"}; copies up to n
  * floats, returns count.  "qcdf": the learned proposal's cumulative weights of the
  * last batch, a row per slot in emitter order (the batch must cover every slot);
  * "features": the proposal's 6 input features per slot of the last run, emitter order. */
@@ -287,6 +287,12 @@ long long dnsmc_get_ray_records(dnsmc_solver *s, int *steps, int *end_type, unsi
                                 float *D, float *G, long long n);
 /* Switch on per-ray records (costs bandwidth; off by default). */
 int dnsmc_set_record(dnsmc_solver *s, int on);
+/* Test hook of the learned estimator's in-traversal draw (no reference
+ * counterpart): with the log on, the run writes every proposal row the waves
+ * evaluate (dnsmc_get_array "lrows": cumulative weights U, emitter order) and
+ * every drawn ray (dnsmc_get_rays: descriptor words, p/q in w), so that the
+ * draws can be replayed from the same rows. */
+int dnsmc_set_draw_log(dnsmc_solver *s, int on);
 /* Training hooks of the learned proposal (no counterpart in the reference; used
  * by dnsmc_amd/learned.py).  Pointers are device pointers.  Parameters are one
  * FP32 vector theta of dnsmc_mlp_nparams(nbin) floats: W1, b1, W2, b2, W3, b3

@@ -232,6 +232,7 @@ def test_snapshotwise_learned_matches_batched(trained, monkeypatch, nsnap, pipes
     net, _, tdir, n = trained
     if pipes is not None:
         monkeypatch.setenv("DNSMC_PIPES", pipes)
+    monkeypatch.setenv("DNSMC_LEARNED_FUSED", "0")
     out = {}
     for sw in ("1", "0"):
         monkeypatch.setenv("DNSMC_SNAPWISE", sw)
@@ -261,6 +262,7 @@ def test_learned_draw_matches_oracle(oracle, trained, monkeypatch):
     bit and p/q to the last bit (the same FP32 operations)."""
     net, _, tdir, n = trained
     monkeypatch.setenv("DNSMC_SNAPWISE", "0")
+    monkeypatch.setenv("DNSMC_LEARNED_FUSED", "0")
     R = 40
     cfg = _small_case(tdir, 2, R, n=n)
     S = dnsmc_amd.Solver(cfg)
@@ -289,6 +291,7 @@ def test_proposal_rows_match_the_network(trained, monkeypatch):
     features: each head's row is the running sum of exp(clamp(z, -60, 60))."""
     net, _, tdir, n = trained
     monkeypatch.setenv("DNSMC_SNAPWISE", "0")
+    monkeypatch.setenv("DNSMC_LEARNED_FUSED", "0")
     cfg = _small_case(tdir, 2, 8, n=n)
     S = dnsmc_amd.Solver(cfg)
     S.load_tables()
@@ -306,6 +309,92 @@ def test_proposal_rows_match_the_network(trained, monkeypatch):
     for lo, hi in ((0, 64), (64, 64 + nbin)):
         want = np.cumsum(e[:, lo:hi], axis=1)
         np.testing.assert_allclose(U[:, lo:hi], want, rtol=2e-4, atol=1e-30)
+
+
+@pytest.mark.parametrize("R", [40, 130])
+def test_fused_learned_draw_matches_oracle(oracle, trained, R):
+    """The learned draw inside the traversal (the default learned path): the
+    proposal rows the waves evaluated and every drawn ray are logged, and the
+    oracle's learned draw (orc_learned_draw) on those rows gives the same
+    descriptor words and p/q bit for bit.  R = 40 puts up to three cells in
+    one 64-ray queue draw, R = 130 splits cells over draws and waves."""
+    net, _, tdir, n = trained
+    cfg = _small_case(tdir, 2, R, n=n)
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    S.set_mlp(*net.host_weights(), mix_eps=0.1)
+    S.set_field(0, _field(77, n))
+    S.set_draw_log(True)
+    S.run()
+    st = S.stats()
+    assert st["n_trace_launches"] == 1 and st["ms_raygen"] == 0.0   # no generator kernel
+    ncell = n[0] * n[1] * n[2]
+    rays = S.rays(ncell * R)
+    U = S.array("lrows").reshape(ncell, -1)
+    p_bg = S.array("p_bg")
+    pcdf = S.array("pcdf")
+    nQ = int(S.array("scalars")[4])
+    S.close()
+    w0, w1, ratio = oracle.learned_draw(U, pcdf, p_bg, np.arange(ncell, dtype=np.int32), R, 64, nQ, 0,
+                                        cfg.seed, np.float32(0.1))
+    np.testing.assert_array_equal(rays["w0"], w0)
+    np.testing.assert_array_equal(rays["w1"], w1)
+    np.testing.assert_array_equal(rays["w"], ratio)
+
+
+def test_fused_learned_rows_match_the_network(oracle, trained):
+    """The rows the traversal's waves evaluate (BF16 pair products in FP32,
+    H1 and H2 rounded to BF16) against the oracle's network in its BF16
+    rounding mode on the same features: the running sums of
+    exp(clamp(z, -60, 60)) of each head."""
+    net, _, tdir, n = trained
+    cfg = _small_case(tdir, 2, 8, n=n)
+    S = dnsmc_amd.Solver(cfg)
+    S.load_tables()
+    S.set_mlp(*net.host_weights(), mix_eps=0.1)
+    S.set_field(0, _field(78, n))
+    S.set_draw_log(True)
+    S.run()
+    S.wait()
+    ncell = n[0] * n[1] * n[2]
+    U = S.array("lrows").reshape(ncell, -1).astype(np.float64)
+    X = S.array("features").reshape(ncell, 6).astype(np.float64)
+    S.close()
+    _, _, Z = oracle.mlp_forward(X, *_oracle_params(net), bf16=True)
+    nbin = net.nbin
+    e = np.exp(np.clip(Z, -60.0, 60.0))
+    for lo, hi in ((0, 64), (64, 64 + nbin)):
+        want = np.cumsum(e[:, lo:hi], axis=1)
+        rel = np.abs(U[:, lo:hi] - want) / want
+        assert np.median(rel) < 1e-5 and np.max(rel) < 1e-2, (np.median(rel), np.max(rel))
+
+
+def test_fused_learned_split_launches_and_snapshots(trained, monkeypatch):
+    """Three snapshots in one fused learned launch against the same solve
+    split into launches of 37 slots: every ray is drawn from its (ray, cell,
+    snapshot) alone, so the step totals are equal and the tallies agree up to
+    the order of the FP64 atomic additions."""
+    net, _, tdir, n = trained
+    out = []
+    for slots in (None, "37"):
+        if slots:
+            monkeypatch.setenv("DNSMC_FUSED_SLOTS", slots)
+        cfg = workloads.channel_config(n, rays_per_cell=48, spectral=0, Tw=573.0, table_dir=tdir, mode=2, nsnap=3)
+        S = dnsmc_amd.Solver(cfg)
+        S.load_tables()
+        S.set_mlp(*net.host_weights(), mix_eps=0.1)
+        for s in range(3):
+            S.set_field(s, _field(90 + s, n))
+        S.run()
+        st = S.stats()
+        out.append((st, [S.results(s) for s in range(3)]))
+        S.close()
+    (s1, r1), (s2, r2) = out
+    assert s1["n_trace_launches"] == 1 and s2["n_trace_launches"] > 1
+    assert s1["n_steps"] == s2["n_steps"]
+    for s in range(3):
+        np.testing.assert_allclose(r1[s]["E"], r2[s]["E"], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(r1[s]["A"], r2[s]["A"], rtol=1e-12, atol=0)
 
 
 def test_learned_estimator_unbiased_on_the_full_channel(trained):
