@@ -191,6 +191,33 @@ def solve_cost(workload, R, fields, mode=0, mlp=None, mix_eps=0.1, **cfg_kw):
     return ts[0] - b * R, b, ts, keep[0], keep[1]
 
 
+def learned_draw_paths(workload, R, fields, W, mix_eps):
+    """One learned solve of snapshot 0 at R with each of the two ways of
+    drawing learned rays: the default (proposal rows of 64 cells per MFMA pass
+    and the draws in generation kernels, overlapped with the traversal of the
+    previous snapshot) and the draw inside the traversal (each wave evaluates
+    its cells' rows itself, DNSMC_LEARNED_FUSED=1)."""
+    from dnsmc_amd import study
+    out = {}
+    for key, env in (("separate_generation", None), ("in_traversal", "1")):
+        if env:
+            os.environ["DNSMC_LEARNED_FUSED"] = env
+        try:
+            S1, _ = study.make_solver(workload, R, mode=2, mlp=W, mix_eps=mix_eps, fields=fields[:1])
+            S1.run()
+            S1.wait()
+            t0 = time.perf_counter()
+            S1.run()
+            S1.wait()
+            out[key] = time.perf_counter() - t0
+            st = S1.stats()
+            out[key + "_trace_launches"] = st["n_trace_launches"]
+            S1.close()
+        finally:
+            os.environ.pop("DNSMC_LEARNED_FUSED", None)
+    return out
+
+
 def accuracy_legs(args, cfg, R, fields):
     """Plain and learned estimators on the fine grid and with the reference's
     default mesh coarsening, all scored against ONE converged FP64 reference
@@ -228,6 +255,8 @@ def accuracy_legs(args, cfg, R, fields):
             if mode == 2:
                 leg["proposal"] = info
                 leg["variance_ratio_vs_plain"] = float(np.sum(legs["plain_" + grid]["_sd2"]) / np.sum(res["sdev"] ** 2))
+                if grid == "fine":
+                    leg["draw_paths_s_snapshot0"] = learned_draw_paths(wl, R, fields, Wm, args.mix_eps)
             leg["_sd2"] = res["sdev"] ** 2
             # rays per cell to 1% RMS, measured by doubling R on the sampled cells
             tt = study.rays_to_target(wl, cells, Qref, vref, fields, mode=mode, mlp=Wm, mix_eps=args.mix_eps,

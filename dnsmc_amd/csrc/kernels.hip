@@ -469,6 +469,19 @@ __device__ __forceinline__ float dot2bf(uint32_t a, uint32_t b, float c)
 
 __device__ __forceinline__ uint32_t readlane_u(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
 
+// The trace kernel's argument block (its only argument, at offset 0 of the
+// kernel argument segment) through a pointer the compiler cannot see through:
+// fields read through it are loaded (scalar loads from the constant segment)
+// where they are used instead of being hoisted to the kernel's start and held
+// in scalar registers through the traversal loop.
+typedef __attribute__((address_space(4))) const TraceArgs KArgs;
+__device__ __forceinline__ KArgs* kernel_args()
+{
+    KArgs* k = (KArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(k));
+    return k;
+}
+
 // The row of emitting cell ijk (i | j << 10 | k << 20) of snapshot
 // `snapshot` into U (all 64 lanes call it): H1 = tanh(X W1 + b1), H2 =
 // tanh(H1 W2 + b2), Z = H2 W3 + b3, H1 and H2 rounded to BF16 as the training
@@ -728,7 +741,6 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     const uint32_t per_x = (cps + nparts - 1u) / nparts;        // per XCD and snapshot
     const uint32_t nclaim = per_x * (uint32_t)P.nsnap_launch;   // claims of one XCD counter
     const int nT = P.nT;
-    const unsigned long long lanemask_lt = (1ull << lane) - 1ull;
 
     // wave-uniform cursor over the claimed super-chunk (rays cur..cend of snapshot csnap)
     uint32_t xi = 0, cur = 0, cend = 0, csnap = 0;
@@ -795,7 +807,6 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     float lr_ratio = 0.f;
     uint32_t row_slot = 0xFFFFFFFFu, row_snap = 0xFFFFFFFFu;
     const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    float* const row_buf = GEN == 2 ? P.lrow_cache + (size_t)wave_id * (size_t)kLRowFloats : nullptr;
 
     // ray q of snapshot qs of the launch: drawn here (GEN, the plain estimator's
     // Philox draw, bit-identical to raygen_plain) or read from its descriptors
@@ -934,7 +945,9 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 steps_lane += (uint32_t)rsteps;
                 fin = false;
             }
-            const uint32_t rank = (uint32_t)__popcll(m_idle & lanemask_lt);
+            // idle lanes below this one (mbcnt: no lane mask held in registers)
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_idle >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)m_idle, 0u));
             // the drawn ray rd of snapshot rsn becomes this lane's ray
             auto start_ray = [&](const RayStart& rd, uint32_t rsn) {
                 rsnap = rsn;
@@ -1033,47 +1046,52 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                                                           (uint32_t)(P.seed >> 32));
                             const uint32_t s0 = readlane_u(slot, 0), s1 = readlane_u(slot, (int)gn - 1);
                             lr_band = 0; lr_g = 0; lr_mt = 0; lr_mp = 0; lr_ratio = 0.f;
+                            // the network's arguments are read from the kernel
+                            // argument segment here, not kept in registers through
+                            // the traversal
+                            const KArgs* K = kernel_args();
+                            float* const row_buf = K->lrow_cache + (size_t)wave_id * (size_t)kLRowFloats;
                             for (uint32_t sl = s0; sl <= s1; ++sl) {
                                 if (sl != row_slot || gsnap != row_snap) {
-                                    const FeatArgs& fa = P.feat;
-                                    learned_row(P.mlpw.W1, P.mlpw.b1, P.mlpw.W2, P.mlpw.b2, P.mlpw.W3, P.mlpw.b3,
-                                                P.mlpw.nbin, P.mlpw.noutp,
-                                                fa.T0 + (size_t)snapshot * (size_t)P.feat_stride, fa.n[0], fa.n[1],
-                                                fa.n[2], fa.nbx, fa.nby, fa.centres, fa.coff[0], fa.coff[1],
-                                                fa.coff[2], fa.bc_periodic, fa.polar, fa.Lpolar, fa.inv_Tmax,
-                                                fa.Tw_rel, P.slotinfo[P.slot0 + sl].w, lane, row_buf);
+                                    learned_row(K->mlpw.W1, K->mlpw.b1, K->mlpw.W2, K->mlpw.b2, K->mlpw.W3, K->mlpw.b3,
+                                                K->mlpw.nbin, K->mlpw.noutp,
+                                                K->feat.T0 + (size_t)snapshot * (size_t)K->feat_stride, K->feat.n[0],
+                                                K->feat.n[1], K->feat.n[2], K->feat.nbx, K->feat.nby, K->feat.centres,
+                                                K->feat.coff[0], K->feat.coff[1], K->feat.coff[2], K->feat.bc_periodic,
+                                                K->feat.polar, K->feat.Lpolar, K->feat.inv_Tmax, K->feat.Tw_rel,
+                                                P.slotinfo[P.slot0 + sl].w, lane, row_buf);
                                     row_slot = __builtin_amdgcn_readfirstlane(sl);
                                     row_snap = __builtin_amdgcn_readfirstlane(gsnap);
                                     // the row's columns were written by every lane of the wave
                                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                                    if (P.log_rows) {
-                                        const int no = P.mlpw.noutp;
-                                        float* lr = P.log_rows + (size_t)(P.slot0 + sl) * (size_t)no;
+                                    if (K->log_rows) {
+                                        const int no = K->mlpw.noutp;
+                                        float* lr = K->log_rows + (size_t)(P.slot0 + sl) * (size_t)no;
                                         for (int cc = lane; cc < no; cc += 64) lr[cc] = row_buf[cc];
                                     }
                                 }
                                 if (slot == sl) {
                                     float qd, qb;
                                     int d, bg;
-                                    mix_search2(row_buf, P.pdir, kMlpDir, row_buf + kMlpDir, P.pcdf, P.mlpw.nbin,
-                                                P.mix_eps, u01_open(x.x), u01_open(x.y), &d, &qd, &bg, &qb);
+                                    mix_search2(row_buf, K->pdir, kMlpDir, row_buf + kMlpDir, K->pcdf, K->mlpw.nbin,
+                                                K->mix_eps, u01_open(x.x), u01_open(x.y), &d, &qd, &bg, &qb);
                                     lr_band = bg / P.nQ;
                                     lr_g = bg - lr_band * P.nQ;
-                                    lr_ratio = (qb > 0.f && qd > 0.f) ? (P.p_bg[bg] / qb) * ((1.0f / kMlpDir) / qd) : 0.f;
+                                    lr_ratio = (qb > 0.f && qd > 0.f) ? (K->p_bg[bg] / qb) * ((1.0f / kMlpDir) / qd) : 0.f;
                                     // direction bin d = 8 cos(theta) bins x 8 phi bins: the bin in
                                     // the top 3 of the 24 uniform bits, the Philox bits below
                                     lr_mt = ((uint32_t)(d >> 3) << 21) | (x.z >> 11);
                                     lr_mp = ((uint32_t)(d & 7) << 21) | (x.w >> 11);
                                 }
                             }
-                            if (P.log_w0 && (uint32_t)lane < gn) {
+                            if (K->log_w0 && (uint32_t)lane < gn) {
                                 uint32_t w0, w1;
                                 desc_pack(lr_mt, lr_mp, lr_band, lr_g, &w0, &w1);
                                 const size_t o = (size_t)gsnap * (size_t)P.nrays + (cur + (uint32_t)lane);
-                                P.log_w0[o] = w0;
-                                P.log_w1[o] = w1;
-                                P.log_w[o] = lr_ratio;
+                                K->log_w0[o] = w0;
+                                K->log_w1[o] = w1;
+                                K->log_w[o] = lr_ratio;
                             }
                         }
                         if ((uint32_t)lane < gn) Qr = draw_ray(cur + (uint32_t)lane, gsnap);

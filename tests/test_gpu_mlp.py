@@ -312,12 +312,13 @@ def test_proposal_rows_match_the_network(trained, monkeypatch):
 
 
 @pytest.mark.parametrize("R", [40, 130])
-def test_fused_learned_draw_matches_oracle(oracle, trained, R):
-    """The learned draw inside the traversal (the default learned path): the
+def test_fused_learned_draw_matches_oracle(oracle, trained, R, monkeypatch):
+    """The learned draw inside the traversal (DNSMC_LEARNED_FUSED=1): the
     proposal rows the waves evaluated and every drawn ray are logged, and the
     oracle's learned draw (orc_learned_draw) on those rows gives the same
     descriptor words and p/q bit for bit.  R = 40 puts up to three cells in
     one 64-ray queue draw, R = 130 splits cells over draws and waves."""
+    monkeypatch.setenv("DNSMC_LEARNED_FUSED", "1")
     net, _, tdir, n = trained
     cfg = _small_case(tdir, 2, R, n=n)
     S = dnsmc_amd.Solver(cfg)
@@ -342,11 +343,12 @@ def test_fused_learned_draw_matches_oracle(oracle, trained, R):
     np.testing.assert_array_equal(rays["w"], ratio)
 
 
-def test_fused_learned_rows_match_the_network(oracle, trained):
+def test_fused_learned_rows_match_the_network(oracle, trained, monkeypatch):
     """The rows the traversal's waves evaluate (BF16 pair products in FP32,
     H1 and H2 rounded to BF16) against the oracle's network in its BF16
     rounding mode on the same features: the running sums of
     exp(clamp(z, -60, 60)) of each head."""
+    monkeypatch.setenv("DNSMC_LEARNED_FUSED", "1")
     net, _, tdir, n = trained
     cfg = _small_case(tdir, 2, 8, n=n)
     S = dnsmc_amd.Solver(cfg)
@@ -374,6 +376,7 @@ def test_fused_learned_split_launches_and_snapshots(trained, monkeypatch):
     split into launches of 37 slots: every ray is drawn from its (ray, cell,
     snapshot) alone, so the step totals are equal and the tallies agree up to
     the order of the FP64 atomic additions."""
+    monkeypatch.setenv("DNSMC_LEARNED_FUSED", "1")
     net, _, tdir, n = trained
     out = []
     for slots in (None, "37"):
@@ -395,6 +398,34 @@ def test_fused_learned_split_launches_and_snapshots(trained, monkeypatch):
     for s in range(3):
         np.testing.assert_allclose(r1[s]["E"], r2[s]["E"], rtol=1e-12, atol=0)
         np.testing.assert_allclose(r1[s]["A"], r2[s]["A"], rtol=1e-12, atol=0)
+
+
+def test_fused_learned_agrees_with_separate_generation(trained, monkeypatch):
+    """The draw inside the traversal against the default generation kernels
+    on the same seed: both draw ray (r, cell, snapshot) from Philox
+    (r, cell, snapshot, 1) on the same network, so the rays differ only where
+    the two row evaluations (per-wave BF16 pair products against 16x16x16
+    MFMA tiles) round a bin edge differently; the step totals agree to 1e-3
+    and the per-cell Q to within one standard deviation (a few cells hold a
+    ray drawn into a neighbouring bin, the rest are identical)."""
+    net, _, tdir, n = trained
+    out = []
+    for env in ("0", "1"):
+        monkeypatch.setenv("DNSMC_LEARNED_FUSED", env)
+        S = dnsmc_amd.Solver(_small_case(tdir, 2, 64, n=n))
+        S.load_tables()
+        S.set_mlp(*net.host_weights(), mix_eps=0.1)
+        S.set_field(0, _field(95, n))
+        S.run()
+        out.append((S.stats(), S.results(0)))
+        S.close()
+    (s0, r0), (s1, r1) = out
+    assert s1["n_trace_launches"] == 1 and s1["ms_raygen"] == 0.0 and s0["ms_raygen"] > 0.0
+    assert s0["n_rays"] == s1["n_rays"]
+    assert abs(s0["n_steps"] / s1["n_steps"] - 1.0) < 1e-3
+    z = (r0["Q"] - r1["Q"]) / np.sqrt(r0["sdev"] ** 2 + r1["sdev"] ** 2 + 1e-300)
+    assert np.max(np.abs(z)) < 1.0, np.max(np.abs(z))
+    assert float(np.std(z)) < 0.05, float(np.std(z))
 
 
 def test_learned_estimator_unbiased_on_the_full_channel(trained):
