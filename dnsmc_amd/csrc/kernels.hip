@@ -1217,18 +1217,24 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         int cm = (m0 ? cx : (m1 ? cy : cz)) + stp;
         const int nm = m0 ? n0 : (m1 ? n1 : n2);
         const int fo = m0 ? fo0 : (m1 ? fo1 : fo2);
-        const bool out = (cm == 0) | (cm == nm + 1);
-        const int side = cm != 0;
-        const int bcbit = ax * 2 + side;
-        const bool per = (P.bc_periodic >> bcbit) & 1;
-        const int wall = (out && !per) ? bcbit : -1;
-        // periodic wrap; a wall hit keeps the old cell (the ray ends there)
-        const bool wrap = out && per;
-        const float pw = s_face[fo + (side ? 0 : nm)];
-        cm = wrap ? (side ? 1 : nm) : (out ? cm - stp : cm);
-        px = (wrap && m0) ? pw : px;
-        py = (wrap && m1) ? pw : py;
-        pz = (wrap && m2) ? pw : pz;
+        // the domain boundary (cm = 0 or nm + 1): a periodic wrap, or a wall
+        // hit that keeps the old cell (the ray ends there).  Rare per lane, so
+        // a branch: most steps of a wave have no lane at the boundary.
+        int wall = -1;
+        if ((uint32_t)(cm - 1) >= (uint32_t)nm) {
+            const int side = cm != 0;
+            const int bcbit = ax * 2 + side;
+            if ((P.bc_periodic >> bcbit) & 1) {
+                const float pw = s_face[fo + (side ? 0 : nm)];
+                cm = side ? 1 : nm;
+                px = m0 ? pw : px;
+                py = m1 ? pw : py;
+                pz = m2 ? pw : pz;
+            } else {
+                wall = bcbit;
+                cm -= stp;
+            }
+        }
         const float fnew = s_face[fo + cm - fla];
         cx = m0 ? cm : cx;
         cy = m1 ? cm : cy;
