@@ -276,4 +276,44 @@ __device__ __forceinline__ int mix_search(const float* Uh, const float* P, int n
     return k;
 }
 
+// mix_search on both heads of a row at once: the two binary searches advance
+// in the same loop (their probes are independent loads), each with exactly
+// mix_search's probe sequence and special cases, so the bins and
+// probabilities are mix_search's.
+__device__ __forceinline__ void mix_search2(const float* Ud, const float* Pd, int nd, const float* Ub,
+                                            const float* Pb, int nb, float eps, float Rd, float Rb, int* kd,
+                                            float* qd, int* kb, float* qb)
+{
+    const float sd = (1.0f - eps) / Ud[nd - 1], sb = (1.0f - eps) / Ub[nb - 1];
+    auto Fd = [&](int k) { return k < 0 ? 0.f : (k == nd - 1 ? 1.0f : fminf(fmaf(Ud[k], sd, eps * Pd[k]), 1.0f)); };
+    auto Fb = [&](int k) { return k < 0 ? 0.f : (k == nb - 1 ? 1.0f : fminf(fmaf(Ub[k], sb, eps * Pb[k]), 1.0f)); };
+    int lod = 0, wd = nd, lob = 0, wb = nb;
+    while (wd > lod + 1 || wb > lob + 1) {
+        const bool ad = wd > lod + 1, ab = wb > lob + 1;
+        const int dd = (lod + wd) >> 1, db = (lob + wb) >> 1;
+        const float fd = ad ? Fd(dd) : 0.f;
+        const float fb = ab ? Fb(db) : 0.f;
+        if (ad) {
+            if (Rd < fd) wd = dd;
+            if (Rd >= fd) lod = dd;
+        }
+        if (ab) {
+            if (Rb < fb) wb = db;
+            if (Rb >= fb) lob = db;
+        }
+    }
+    int k = lod + 1;
+    if (Rd < Fd(0)) k = 0;
+    if (Rd == 1.0f) k = nd - 1;
+    if (k > nd - 1) k = nd - 1;
+    *qd = Fd(k) - Fd(k - 1);
+    *kd = k;
+    k = lob + 1;
+    if (Rb < Fb(0)) k = 0;
+    if (Rb == 1.0f) k = nb - 1;
+    if (k > nb - 1) k = nb - 1;
+    *qb = Fb(k) - Fb(k - 1);
+    *kb = k;
+}
+
 }  // namespace dn

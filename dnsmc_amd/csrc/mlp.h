@@ -44,7 +44,21 @@ struct MlpWeights {
     const uint16_t* W3;   // [64][noutp] bf16
     const float* b3;      // [noutp]
     int nbin, noutp;
+    // the same W2 and W3 with the input index contiguous ([64][64] and
+    // [noutp][64]), for the proposal kernel's MFMA operand tiles (set by the
+    // solver; the training kernels do not use them)
+    const uint16_t* W2T;
+    const uint16_t* W3T;
 };
+
+// Row length (elements) of the proposal kernel's LDS tiles: 68 BF16 = 34
+// banks, so the 16 rows one MFMA operand fragment reads start in 16
+// different even banks, and 68 floats put rows 4 apart 16 banks apart.
+constexpr int kPropStride = 68;
+__host__ __device__ inline size_t mlp_proposal_lds()
+{
+    return sizeof(float) * (64 * kMlpIn + 64 * kPropStride) + 2 * 64 * kPropStride * sizeof(uint16_t);
+}
 
 struct MlpTrainArgs {
     const float* X;       // [n][6]

@@ -84,7 +84,8 @@ __device__ __forceinline__ void acc_each(const f4v acc[2][2], F f)
 }
 
 // H1 = tanh(X W1 + b1) for rows < nrow (zero rows after), as FP32 and BF16.
-__device__ __forceinline__ void layer1(const float* sX, const MlpWeights& w, int nrow, float* H1f, uint16_t* H1b)
+__device__ __forceinline__ void layer1(const float* sX, const MlpWeights& w, int nrow, float* H1f, uint16_t* H1b,
+                                       int hs = kMlpHid)
 {
     for (int e = threadIdx.x; e < 64 * kMlpHid; e += blockDim.x) {
         const int s = e / kMlpHid, j = e - s * kMlpHid;
@@ -95,7 +96,7 @@ __device__ __forceinline__ void layer1(const float* sX, const MlpWeights& w, int
             h = tanhf(acc);
         }
         if (H1f) H1f[e] = h;
-        H1b[e] = f2bf(h);
+        H1b[s * hs + j] = f2bf(h);
     }
 }
 
@@ -108,6 +109,38 @@ __device__ __forceinline__ void stage_w3_tile(const uint16_t* W3, int noutp, int
     for (int e = threadIdx.x; e < 512; e += blockDim.x) {
         const int h = e >> 3, c8 = e & 7;
         d[e] = src[((size_t)h * noutp + jt * 64) / 8 + c8];
+    }
+}
+
+// Barrier for the block's LDS traffic only: each wave's LDS operations are
+// complete before it, its global stores (the rows, read by no other thread
+// of this kernel) stay in flight.
+__device__ __forceinline__ void lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Tile jt of W3^T (rows jt*64 .. jt*64 + 63 of the [noutp][64] copy, 128 B
+// each) for a 256-thread block in two steps: its two 16-byte pieces per
+// thread into registers (issued one tile ahead), then into the padded LDS
+// rows (8-byte writes: the padded rows are 8-byte aligned).
+__device__ __forceinline__ void load_w3_tile(const uint16_t* W3T, int jt, uint4& r0, uint4& r1)
+{
+    const uint4* src = (const uint4*)(W3T + (size_t)jt * 64 * 64);
+    r0 = src[threadIdx.x];
+    r1 = src[threadIdx.x + 256];
+}
+__device__ __forceinline__ void put_w3_tile(const uint4& r0, const uint4& r1, uint16_t* dst)
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int e = (int)threadIdx.x + 256 * i, n = e >> 3, k8 = e & 7;
+        const uint4 r = i ? r1 : r0;
+        uint2* d = (uint2*)(dst + n * kPropStride + k8 * 8);
+        d[0] = make_uint2(r.x, r.y);
+        d[1] = make_uint2(r.z, r.w);
     }
 }
 
@@ -450,57 +483,78 @@ __global__ void mlp_features(FeatArgs f, const int4* slotinfo, long long n, floa
 // mixture on the fly, so no second pass over the logits is needed.
 __global__ __launch_bounds__(256) void mlp_proposal(GenArgs a, float* U)
 {
+    // LDS (mlp_proposal_lds()): rows of kPropStride elements, so that MFMA
+    // operand fragments (four consecutive BF16 of one row, one 8-byte read)
+    // and the row scans are free of bank conflicts
     extern __shared__ float lds[];
-    float* sX = lds;                                  // 64*6
-    float* sZ = sX + 64 * kMlpIn;                     // 64*64
-    float* srun = sZ + 4096;                          // 64 running sums
-    uint16_t* sH1b = (uint16_t*)(srun + 64);
-    uint16_t* sH2b = sH1b + 4096;
-    uint16_t* sW2b = sH2b + 4096;
-    uint16_t* sW3b = sW2b + 4096;
+    constexpr int PS = kPropStride;
+    float* sX = lds;                                  // 64 x 6 features
+    float* sZ = sX + 64 * kMlpIn;                     // 64 x PS: exp(z) of a column tile
+    uint16_t* sW2T = (uint16_t*)sZ;                   //   (W2^T during layer 2)
+    uint16_t* sH1b = (uint16_t*)(sZ + 64 * PS);       // 64 x PS: H1, then the W3^T tile
+    uint16_t* sW3T = sH1b;
+    uint16_t* sH2b = sH1b + 64 * PS;                  // 64 x PS: H2
     const int noutp = a.mlpw.noutp, nbin = a.mlpw.nbin;
     const int tid = threadIdx.x;
     FeatArgs f = a.feat;
     f.T0 = a.T0 + (size_t)a.snap * a.tstride0;
-    for (int e = tid; e < 4096; e += blockDim.x) sW2b[e] = a.mlpw.W2[e];
     const int row = tid >> 2, part = tid & 3;         // 4 threads per row, 16 columns each
     const long long ngroups = (a.nslots + 63) / 64;
     for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
         const long long s0 = grp * 64;
         const int ns = (int)((a.nslots - s0) < 64 ? (a.nslots - s0) : 64);
         for (int e = tid; e < 64 * kMlpIn; e += blockDim.x) sX[e] = 0.f;
+        // W2^T (64 rows of 64 BF16) into the padded rows of the sZ region
+        for (int e = tid; e < 512; e += blockDim.x) {
+            const uint4 r = ((const uint4*)a.mlpw.W2T)[e];
+            uint2* d = (uint2*)(sW2T + (e >> 3) * PS + (e & 7) * 8);
+            d[0] = make_uint2(r.x, r.y);
+            d[1] = make_uint2(r.z, r.w);
+        }
         __syncthreads();
         if (tid < ns) {
             const int4 si = a.slotinfo[a.slot0 + s0 + tid];
             cell_features(f, si.w & 1023, (si.w >> 10) & 1023, (si.w >> 20) & 1023, sX + tid * kMlpIn);
         }
         __syncthreads();
-        layer1(sX, a.mlpw, ns, nullptr, sH1b);
+        layer1(sX, a.mlpw, ns, nullptr, sH1b, PS);
         __syncthreads();
         {
             f4v acc[2][2];
             acc_zero(acc);
-            gemm64(sH1b, 64, 1, sW2b, 64, 1, 64, acc);
-            acc_each(acc, [&](int r, int c, float v) { sH2b[r * 64 + c] = f2bf(r < ns ? tanhf(v + a.mlpw.b2[c]) : 0.f); });
+            gemm64(sH1b, PS, 1, sW2T, 1, PS, 64, acc);
+            acc_each(acc, [&](int r, int c, float v) { sH2b[r * PS + c] = f2bf(r < ns ? tanhf(v + a.mlpw.b2[c]) : 0.f); });
         }
         __syncthreads();
         float run = 0.f;                              // this row's running sum (held by all 4 lanes)
+        // the W3 tile and this lane's two b3 columns of it (qc + lane % 16 and
+        // 16 more) are loaded one tile ahead; the tile loop's barriers wait
+        // for LDS traffic only
+        const int lane = tid & 63, bc = ((tid >> 6) & 1) * 32 + (lane & 15);
+        uint4 w3a, w3b;
+        load_w3_tile(a.mlpw.W3T, 0, w3a, w3b);
+        float b3a = a.mlpw.b3[bc], b3b = a.mlpw.b3[bc + 16];
         for (int jt = 0; jt < noutp / 64; ++jt) {
-            stage_w3_tile(a.mlpw.W3, noutp, jt, sW3b);
-            __syncthreads();
+            put_w3_tile(w3a, w3b, sW3T);
+            const float ba = b3a, bb = b3b;
+            lds_sync();
+            const int jn = min(jt + 1, noutp / 64 - 1);   // (the last tile again at the end)
+            load_w3_tile(a.mlpw.W3T, jn, w3a, w3b);
+            b3a = a.mlpw.b3[jn * 64 + bc];
+            b3b = a.mlpw.b3[jn * 64 + bc + 16];
             {
                 f4v acc[2][2];
                 acc_zero(acc);
-                gemm64(sH2b, 64, 1, sW3b, 64, 1, 64, acc);
+                gemm64(sH2b, PS, 1, sW3T, 1, PS, 64, acc);
                 acc_each(acc, [&](int r, int c, float v) {
                     const int gc = jt * 64 + c;
-                    const float z = fminf(fmaxf(v + a.mlpw.b3[gc], -60.f), 60.f);
-                    sZ[r * 64 + c] = gc < kMlpDir + nbin ? __expf(z) : 0.f;
+                    const float z = fminf(fmaxf(v + ((c & 16) ? bb : ba), -60.f), 60.f);
+                    sZ[r * PS + c] = gc < kMlpDir + nbin ? __expf(z) : 0.f;
                 });
             }
-            __syncthreads();
+            lds_sync();
             if (jt == 1) run = 0.f;                   // the (band, g) head starts
-            float* zr = sZ + row * 64 + part * 16;
+            const float* zr = sZ + row * PS + part * 16;
             float loc[16];
             float acc = 0.f;
 #pragma unroll
@@ -516,12 +570,14 @@ __global__ __launch_bounds__(256) void mlp_proposal(GenArgs a, float* U)
             const float total = __shfl(excl, 3, 4);
             excl -= acc;
             if (row < ns) {
-                float* dst = U + (s0 + row) * (size_t)noutp + jt * 64 + part * 16;
+                float4* dst = (float4*)(U + (s0 + row) * (size_t)noutp + jt * 64 + part * 16);
+                const float b0 = run + excl;
 #pragma unroll
-                for (int c = 0; c < 16; ++c) dst[c] = run + excl + loc[c];
+                for (int c = 0; c < 4; ++c)
+                    dst[c] = make_float4(b0 + loc[4 * c], b0 + loc[4 * c + 1], b0 + loc[4 * c + 2], b0 + loc[4 * c + 3]);
             }
             run += total;
-            __syncthreads();
+            lds_sync();
         }
     }
 }
@@ -548,9 +604,18 @@ __global__ __launch_bounds__(256) void raygen_learned(GenArgs a, const float* U,
     for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
         const long long s0 = grp * 8;
         const int ns = (int)((a.nslots - s0) < 8 ? (a.nslots - s0) : 8);
+        // the rows into LDS, eight 16-byte loads in flight per thread
         const float4* src = (const float4*)(U + s0 * (size_t)noutp);
         float4* dst = (float4*)lds;
-        for (int e = threadIdx.x; e < ns * noutp / 4; e += blockDim.x) dst[e] = src[e];
+        const int nv = ns * noutp / 4;
+        for (int e0 = (int)threadIdx.x; e0 < nv; e0 += 8 * (int)blockDim.x) {
+            // clamped (always valid) indices, no branch between the loads
+            float4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = src[min(e0 + u * (int)blockDim.x, nv - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) dst[min(e0 + u * (int)blockDim.x, nv - 1)] = v[u];   // (clamped: the same value)
+        }
         if (threadIdx.x < ns) s_cell[threadIdx.x] = a.slotinfo[a.slot0 + s0 + threadIdx.x].y;
         __syncthreads();
         for (int t = threadIdx.x; t < ns * R; t += blockDim.x) {

@@ -235,6 +235,7 @@ struct dnsmc_solver {
     int mlp_nbin = 0;
     float mix_eps = 0.1f;
     DevBuf<uint16_t> d_W1b, d_W2b, d_W3b;
+    DevBuf<uint16_t> d_W2T, d_W3T;   // W2, W3 with the input index contiguous (the proposal kernel's operands)
     DevBuf<float> d_b1, d_b2, d_b3, d_pbg, d_pcdf;
     DevBuf<float> d_lcache;    // learned draw in the traversal: each wave's row buffer
     DevBuf<float> d_pdir;      // the direction head's plain CDF
@@ -648,6 +649,8 @@ static MlpWeights mlp_weights(dnsmc_solver* s)
     w.W3 = s->d_W3b.p; w.b3 = s->d_b3.p;
     w.nbin = s->mlp_nbin;
     w.noutp = mlp_noutp(s->mlp_nbin);
+    w.W2T = s->d_W2T.p;
+    w.W3T = s->d_W3T.p;
     return w;
 }
 
@@ -1109,7 +1112,7 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
         // generation on pipe 1 into descriptor set (snap % 2), one traversal
         // launch per snapshot on the main stream
         hipStream_t gs = s->pipe[1].stream, ts = s->pipe[0].stream;
-        const size_t lds_prop = sizeof(float) * (64 * kMlpIn + 4096 + 64) + 4 * 4096 * sizeof(uint16_t);
+        const size_t lds_prop = mlp_proposal_lds();
         for (int snap = 0; snap < c.nsnap; ++snap) {
             const int b = snap & 1;
             dnsmc_solver::Pipe& buf = s->pipe[b];
@@ -1173,7 +1176,7 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
             EventPair& eg = next_event(s, 0);
             HIPCHK(hipEventRecord(eg.a, st));
             if (c.mode == DNSMC_MODE_LEARNED) {
-                const size_t lds_prop = sizeof(float) * (64 * kMlpIn + 4096 + 64) + 4 * 4096 * sizeof(uint16_t);
+                const size_t lds_prop = mlp_proposal_lds();
                 int nb = (int)std::min<long long>((nsl + 63) / 64, (long long)s->ncu * 3);
                 hipLaunchKernelGGL(mlp_proposal, dim3(nb), dim3(256), lds_prop, st, G, pp.d_qcdf.p);
                 nb = (int)std::min<long long>((nsl + 7) / 8, (long long)s->ncu * 3);
@@ -1644,6 +1647,16 @@ int dnsmc_set_mlp(dnsmc_solver* s, const unsigned short* W1, const float* b1, co
     HIPCHK(up32(s->d_b2, b2, kMlpHid));
     HIPCHK(up16(s->d_W3b, W3, (size_t)kMlpHid * noutp));
     HIPCHK(up32(s->d_b3, b3, noutp));
+    {   // the transposed copies: [out][in]
+        std::vector<uint16_t> t2((size_t)kMlpHid * kMlpHid), t3((size_t)noutp * kMlpHid);
+        for (int k = 0; k < kMlpHid; ++k)
+            for (int n = 0; n < kMlpHid; ++n) t2[(size_t)n * kMlpHid + k] = W2[(size_t)k * kMlpHid + n];
+        for (int k = 0; k < kMlpHid; ++k)
+            for (int n = 0; n < noutp; ++n) t3[(size_t)n * kMlpHid + k] = W3[(size_t)k * noutp + n];
+        HIPCHK(s->d_W2T.upload(t2, s->stream));
+        HIPCHK(s->d_W3T.upload(t3, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));   // (t2, t3 are read until the copies end)
+    }
     HIPCHK(hipStreamSynchronize(s->stream));
     s->have_mlp = true;
     s->mlp_nbin = nbin;
@@ -2082,6 +2095,8 @@ static MlpWeights mlp_weights_of(int nbin, const unsigned short* W1b, const unsi
     w.b1 = theta + L.ob1; w.b2 = theta + L.ob2; w.b3 = theta + L.ob3;
     w.nbin = nbin;
     w.noutp = L.noutp;
+    w.W2T = nullptr;   // (the training and row-forward kernels read W2, W3)
+    w.W3T = nullptr;
     return w;
 }
 

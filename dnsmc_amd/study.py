@@ -21,6 +21,7 @@ max |Q - Q_ref| / max |Q_ref|.
 """
 import json
 import math
+import os
 import sys
 import time
 
@@ -267,25 +268,39 @@ def study_heldout(rays=(100, 1000), snaps=range(51, 56), mix_eps=0.05, **cfg_kw)
     return out
 
 
-def study_legs(workload="ck384x8", rays=100):
-    """One snapshot of the workload solved three ways (plain on the fine grid,
-    plain with the reference's default coarsening, learned on the fine grid):
-    seconds, segments and rays of each solve (the counter profiles of the
-    three traversal kernels are taken around this command)."""
+def study_legs(workload="ck384x8", rays=100, mix_eps=0.1):
+    """One snapshot of the workload solved five ways (plain on the fine grid,
+    plain with the reference's default coarsening, learned on the fine grid
+    with the proposal evaluated by the generation kernels and with the draw
+    inside the traversal (DNSMC_LEARNED_FUSED=1), learned with coarsening),
+    each after one warm-up solve: seconds, segments and rays of each solve
+    (the kernel-trace and counter profiles of the traversal kernels are taken
+    around this command)."""
     from .learned import train_channel_proposal
     fields = workloads.workload_fields(workload, 1)
     amc = dict(grid_num=5, maxs=(5, 5, 5, 5, 100000, 100000))
-    W, _ = train_channel_proposal(n_snap=4)
+    W, _ = train_channel_proposal()
+    Wc, _ = train_channel_proposal(**amc)
     out = {}
-    for name, mode, kw, w in (("plain_fine", 0, {}, None), ("plain_coarsened", 0, amc, None),
-                              ("learned_fine", 2, {}, W)):
-        S, _ = make_solver(workload, rays, mode=mode, mlp=w, mix_eps=0.05, fields=fields, **kw)
-        t0 = time.time()
-        S.run()
-        st = S.stats()
-        out[name] = dict(seconds=time.time() - t0, segments=st["n_steps"], rays=st["n_rays"])
-        log(name, out[name])
-        S.close()
+    for name, mode, kw, w, fused in (("plain_fine", 0, {}, None, None), ("plain_coarsened", 0, amc, None, None),
+                                     ("learned_fine", 2, {}, W, None), ("learned_fine_in_traversal", 2, {}, W, "1"),
+                                     ("learned_coarsened", 2, amc, Wc, None)):
+        if fused:
+            os.environ["DNSMC_LEARNED_FUSED"] = fused
+        try:
+            S, _ = make_solver(workload, rays, mode=mode, mlp=w, mix_eps=mix_eps, fields=fields, **kw)
+            S.run()
+            S.wait()
+            t0 = time.time()
+            S.run()
+            S.wait()
+            st = S.stats()
+            out[name] = dict(seconds=time.time() - t0, segments=st["n_steps"], rays=st["n_rays"],
+                             trace_launches=st["n_trace_launches"], ms_raygen=st["ms_raygen"])
+            log(name, out[name])
+            S.close()
+        finally:
+            os.environ.pop("DNSMC_LEARNED_FUSED", None)
     return out
 
 
