@@ -582,63 +582,47 @@ __global__ __launch_bounds__(256) void mlp_proposal(GenArgs a, float* U)
     }
 }
 
-// raygen_learned: one block per 8 consecutive slots.  Their rows of U are
-// staged in LDS with the plain (band, g) CDF; each ray draws Philox
-// (r, cell, snapshot, 1), its direction bin and (band, g) bin from the
-// defensive mixtures and its direction uniform inside the bin.  The
-// descriptor's w word carries p/q; the traversal multiplies it into the
-// emission weight it computes from its LDS tables at ray start.
+// raygen_learned: one thread per ray.  Each ray draws Philox (r, cell,
+// snapshot, 1), its direction bin and (band, g) bin from the defensive
+// mixtures by binary search on its cell's row of U (global memory: the R rays
+// of a cell run in neighbouring lanes, so their first probes coincide and the
+// rest hit the caches) and its direction uniform inside the bin.  The plain
+// (band, g) CDF is in LDS.  The descriptor's w word carries p/q; the
+// traversal multiplies it into the emission weight it computes from its LDS
+// tables at ray start.
 __global__ __launch_bounds__(256) void raygen_learned(GenArgs a, const float* U, const float* pcdf)
 {
     extern __shared__ float lds[];
     const int noutp = a.mlpw.noutp, nbin = a.mlpw.nbin;
-    float* sP = lds + 8 * noutp;
+    float* sP = lds;
     for (int e = threadIdx.x; e < nbin; e += blockDim.x) sP[e] = pcdf[e];
     __shared__ float sPd[kMlpDir];
     for (int e = threadIdx.x; e < kMlpDir; e += blockDim.x) sPd[e] = (float)(e + 1) * (1.0f / kMlpDir);
-    __shared__ int s_cell[8];
+    __syncthreads();
     const float eps = a.mix_eps;
     const int R = a.rays_per_slot;
-    const long long ngroups = (a.nslots + 7) / 8;
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
-    for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-        const long long s0 = grp * 8;
-        const int ns = (int)((a.nslots - s0) < 8 ? (a.nslots - s0) : 8);
-        // the rows into LDS, eight 16-byte loads in flight per thread
-        const float4* src = (const float4*)(U + s0 * (size_t)noutp);
-        float4* dst = (float4*)lds;
-        const int nv = ns * noutp / 4;
-        for (int e0 = (int)threadIdx.x; e0 < nv; e0 += 8 * (int)blockDim.x) {
-            // clamped (always valid) indices, no branch between the loads
-            float4 v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = src[min(e0 + u * (int)blockDim.x, nv - 1)];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) dst[min(e0 + u * (int)blockDim.x, nv - 1)] = v[u];   // (clamped: the same value)
-        }
-        if (threadIdx.x < ns) s_cell[threadIdx.x] = a.slotinfo[a.slot0 + s0 + threadIdx.x].y;
-        __syncthreads();
-        for (int t = threadIdx.x; t < ns * R; t += blockDim.x) {
-            const int sl = t / R, r = t - sl * R;
-            const u32x4 x = philox4x32_10(u32x4{(uint32_t)r, (uint32_t)s_cell[sl], (uint32_t)a.snap, 1u}, k0, k1);
-            const float* Ud = lds + sl * noutp;
-            float qd, qb;
-            const int d = mix_search(Ud, sPd, kMlpDir, eps, u01_open(x.x), &qd);
-            const int bg = mix_search(Ud + kMlpDir, sP, nbin, eps, u01_open(x.y), &qb);
-            const int band = bg / a.nQ, g = bg - band * a.nQ;
-            // (a selected bin always has a positive step; the guard keeps a
-            // degenerate row from producing inf * 0)
-            const float ratio = (qb > 0.f && qd > 0.f) ? (a.p_bg[bg] / qb) * ((1.0f / kMlpDir) / qd) : 0.f;
-            const uint32_t mt = ((uint32_t)(d >> 3) << 21) | (x.z >> 11);
-            const uint32_t mp = ((uint32_t)(d & 7) << 21) | (x.w >> 11);
-            uint32_t w0, w1;
-            desc_pack(mt, mp, band, g, &w0, &w1);
-            const long long q = (s0 + sl) * (long long)R + r;
-            a.w0[q] = w0;
-            a.w1[q] = w1;
-            a.w[q] = ratio;
-        }
-        __syncthreads();
+    const long long nr = a.nslots * (long long)R;
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nr; q += (long long)gridDim.x * blockDim.x) {
+        const long long sl = q / R;
+        const int r = (int)(q - sl * R);
+        const int cell = a.slotinfo[a.slot0 + sl].y;
+        const u32x4 x = philox4x32_10(u32x4{(uint32_t)r, (uint32_t)cell, (uint32_t)a.snap, 1u}, k0, k1);
+        const float* Ud = U + sl * (size_t)noutp;
+        float qd, qb;
+        const int d = mix_search(Ud, sPd, kMlpDir, eps, u01_open(x.x), &qd);
+        const int bg = mix_search(Ud + kMlpDir, sP, nbin, eps, u01_open(x.y), &qb);
+        const int band = bg / a.nQ, g = bg - band * a.nQ;
+        // (a selected bin always has a positive step; the guard keeps a
+        // degenerate row from producing inf * 0)
+        const float ratio = (qb > 0.f && qd > 0.f) ? (a.p_bg[bg] / qb) * ((1.0f / kMlpDir) / qd) : 0.f;
+        const uint32_t mt = ((uint32_t)(d >> 3) << 21) | (x.z >> 11);
+        const uint32_t mp = ((uint32_t)(d & 7) << 21) | (x.w >> 11);
+        uint32_t w0, w1;
+        desc_pack(mt, mp, band, g, &w0, &w1);
+        a.w0[q] = w0;
+        a.w1[q] = w1;
+        a.w[q] = ratio;
     }
 }
 

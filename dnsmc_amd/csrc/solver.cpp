@@ -1034,7 +1034,7 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
         G.feat = feat_args(s, d);
         G.p_bg = s->d_pbg.p;
         G.mix_eps = s->mix_eps;
-        lds_learn = sizeof(float) * (8 * (size_t)G.mlpw.noutp + G.mlpw.nbin);
+        lds_learn = sizeof(float) * (size_t)G.mlpw.nbin;   // raygen_learned: the plain (band, g) CDF
         if (lds_learn > s->lds_max) return fail(DNSMC_EINVAL, "too many (band, g) bins for the learned proposal");
     }
 
@@ -1130,7 +1130,7 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
                 G.winv = nullptr;
                 int nb = (int)std::min<long long>((nsl + 63) / 64, (long long)s->ncu * 3);
                 hipLaunchKernelGGL(mlp_proposal, dim3(nb), dim3(256), lds_prop, gs, G, s->pipe[1].d_qcdf.p);
-                nb = (int)std::min<long long>((nsl + 7) / 8, (long long)s->ncu * 3);
+                nb = (int)std::min<long long>((nsl * R + 255) / 256, (long long)s->ncu * 16);
                 hipLaunchKernelGGL(raygen_learned, dim3(nb), dim3(256), lds_learn, gs, G,
                                    (const float*)s->pipe[1].d_qcdf.p, (const float*)s->d_pcdf.p);
                 HIPCHK(hipGetLastError());
@@ -1179,7 +1179,7 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
                 const size_t lds_prop = mlp_proposal_lds();
                 int nb = (int)std::min<long long>((nsl + 63) / 64, (long long)s->ncu * 3);
                 hipLaunchKernelGGL(mlp_proposal, dim3(nb), dim3(256), lds_prop, st, G, pp.d_qcdf.p);
-                nb = (int)std::min<long long>((nsl + 7) / 8, (long long)s->ncu * 3);
+                nb = (int)std::min<long long>((nsl * R + 255) / 256, (long long)s->ncu * 16);
                 hipLaunchKernelGGL(raygen_learned, dim3(nb), dim3(256), lds_learn, st, G, (const float*)pp.d_qcdf.p,
                                    (const float*)s->d_pcdf.p);
             } else if (sorted) {
