@@ -170,13 +170,18 @@ def timed_solve(S, steps):
 
 
 def solve_cost(workload, R, fields, mode=0, mlp=None, mix_eps=0.1, **cfg_kw):
-    """One full-grid solve of snapshot 0 at R and at 4R rays per cell (each
-    after a warm-up solve): the fixed part a and the per-ray-per-cell part b
-    of t(R) = a + b R, the two times, and the stats and results of the R solve."""
+    """Full-grid solves of snapshot 0 at R, 4R and 8R rays per cell (each after
+    a warm-up solve): the fixed part a and the per-ray-per-cell part b of
+    t(R') = a + b R' fitted on 4R and 8R, the three times, and the stats and
+    results of the R solve.  The fit uses the larger two because the rays per
+    cell that 1 % RMS needs (10^5 and more) run the way those do: above
+    2^32 - 2^20 rays per 384x256x256 snapshot (170 rays/cell) a solve is cut
+    into several launches, and the learned generation overlaps the traversal
+    of the previous batch instead of preceding a snapshot's one launch."""
     from dnsmc_amd import study
     ts = []
     keep = None
-    for RR in (R, 4 * R):
+    for RR in (R, 4 * R, 8 * R):
         S1, _ = study.make_solver(workload, RR, mode=mode, mlp=mlp, mix_eps=mix_eps, fields=fields[:1], **cfg_kw)
         S1.run()
         S1.wait()
@@ -187,8 +192,8 @@ def solve_cost(workload, R, fields, mode=0, mlp=None, mix_eps=0.1, **cfg_kw):
         if keep is None:
             keep = (S1.stats(), S1.results(0))
         S1.close()
-    b = (ts[1] - ts[0]) / (3 * R)
-    return ts[0] - b * R, b, ts, keep[0], keep[1]
+    b = (ts[2] - ts[1]) / (4 * R)
+    return ts[1] - b * 4 * R, b, ts, keep[0], keep[1]
 
 
 def learned_draw_paths(workload, R, fields, W, mix_eps):
@@ -242,7 +247,7 @@ def accuracy_legs(args, cfg, R, fields):
         for est, mode in (("plain", 0), ("learned", 2)):
             name = "%s_%s" % (est, grid)
             Wm = W if mode == 2 else None
-            # snapshot 0 on the full grid at R and 4R: cost model, rays/s, results
+            # snapshot 0 on the full grid at R, 4R and 8R: cost model, rays/s, results
             a, b, ts, st0, res = solve_cost(wl, R, fields, mode=mode, mlp=Wm, mix_eps=args.mix_eps, **kw)
             e = study.errors(res["Q"][cells], Qref, vref)
             leg = dict(rms_at_bench_R=e["rms"], max_at_bench_R=e["max"],
@@ -250,7 +255,7 @@ def accuracy_legs(args, cfg, R, fields):
                                                        float(np.sum(Qref ** 2))),
                        rays_per_s_snapshot0=st0["n_rays"] / ts[0],
                        segments_per_ray=st0["n_steps"] / max(st0["n_rays"], 1),
-                       solve_s_snapshot0={str(R): ts[0], str(4 * R): ts[1]},
+                       solve_s_snapshot0={str(R): ts[0], str(4 * R): ts[1], str(8 * R): ts[2]},
                        cost_model_s=dict(fixed=a, per_ray_per_cell=b))
             if mode == 2:
                 leg["proposal"] = info
@@ -294,7 +299,7 @@ def accuracy_legs(args, cfg, R, fields):
              "fine-grid FP64 reference, interpolated between the two doublings that bracket 1%% (each leg's "
              "to_1pct holds the points, the first R below 1%% with its error, and the fit eps^2 = b^2 + c/R whose b "
              "is the coarsening bias); time = %d snapshots x (fixed + per-ray cost of a full-grid solve, "
-             "measured at R and 4R)" % (R, len(cells), cfg.nsnap))
+             "fitted on full-grid solves at 4R and 8R, the regime of the large rays per cell)" % (R, len(cells), cfg.nsnap))
     return out
 
 
