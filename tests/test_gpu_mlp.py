@@ -400,19 +400,23 @@ def test_fused_learned_split_launches_and_snapshots(trained, monkeypatch):
         np.testing.assert_allclose(r1[s]["A"], r2[s]["A"], rtol=1e-12, atol=0)
 
 
-def test_fused_learned_agrees_with_separate_generation(trained, monkeypatch):
+@pytest.mark.parametrize("coarse", [False, True])
+def test_fused_learned_agrees_with_separate_generation(trained, monkeypatch, coarse):
     """The draw inside the traversal against the default generation kernels
-    on the same seed: both draw ray (r, cell, snapshot) from Philox
-    (r, cell, snapshot, 1) on the same network, so the rays differ only where
-    the two row evaluations (per-wave BF16 pair products against 16x16x16
-    MFMA tiles) round a bin edge differently; the step totals agree to 1e-3
-    and the per-cell Q to within one standard deviation (a few cells hold a
-    ray drawn into a neighbouring bin, the rest are identical)."""
+    on the same seed, on the fine grid and with two coarsening levels: both
+    draw ray (r, cell, snapshot) from Philox (r, cell, snapshot, 1) on the
+    same network, so the rays differ only where the two row evaluations
+    (per-wave BF16 pair products against 16x16x16 MFMA tiles) round a bin
+    edge differently; the step totals agree to 1e-3 and the per-cell Q to
+    within one standard deviation (a few cells hold a ray drawn into a
+    neighbouring bin, the rest are identical)."""
     net, _, tdir, n = trained
+    kw = dict(grid_num=2, maxs=(4, 100000, 100000, 100000, 100000, 100000)) if coarse else {}
     out = []
     for env in ("0", "1"):
         monkeypatch.setenv("DNSMC_LEARNED_FUSED", env)
-        S = dnsmc_amd.Solver(_small_case(tdir, 2, 64, n=n))
+        S = dnsmc_amd.Solver(workloads.channel_config(n, rays_per_cell=64, spectral=0, Tw=573.0, table_dir=tdir,
+                                                      mode=2, **kw))
         S.load_tables()
         S.set_mlp(*net.host_weights(), mix_eps=0.1)
         S.set_field(0, _field(95, n))
