@@ -626,4 +626,61 @@ __global__ __launch_bounds__(256) void raygen_learned(GenArgs a, const float* U,
     }
 }
 
+// raygen_learned_lds: the same draws with the rows of G consecutive slots
+// staged in LDS per block (G = 1..8 by the host: at 200 rays per cell and
+// more the staging is shared by enough rays that the LDS probes' short
+// latency pays for it).  Rays t of the group: slot t / R, ray t % R.
+__global__ __launch_bounds__(256) void raygen_learned_lds(GenArgs a, const float* U, const float* pcdf, int G)
+{
+    extern __shared__ float lds[];
+    const int noutp = a.mlpw.noutp, nbin = a.mlpw.nbin;
+    float* sP = lds + G * noutp;
+    for (int e = threadIdx.x; e < nbin; e += blockDim.x) sP[e] = pcdf[e];
+    __shared__ float sPd[kMlpDir];
+    for (int e = threadIdx.x; e < kMlpDir; e += blockDim.x) sPd[e] = (float)(e + 1) * (1.0f / kMlpDir);
+    __shared__ int s_cell[8];
+    const float eps = a.mix_eps;
+    const int R = a.rays_per_slot;
+    const long long ngroups = (a.nslots + G - 1) / G;
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        const long long s0 = grp * G;
+        const int ns = (int)((a.nslots - s0) < G ? (a.nslots - s0) : G);
+        // the rows into LDS, eight 16-byte loads in flight per thread
+        // (clamped, always valid indices: no branch between the loads; the
+        // clamped stores repeat the last element's value)
+        const float4* src = (const float4*)(U + s0 * (size_t)noutp);
+        float4* dst = (float4*)lds;
+        const int nv = ns * noutp / 4;
+        for (int e0 = (int)threadIdx.x; e0 < nv; e0 += 8 * (int)blockDim.x) {
+            float4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = src[min(e0 + u * (int)blockDim.x, nv - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) dst[min(e0 + u * (int)blockDim.x, nv - 1)] = v[u];
+        }
+        if ((int)threadIdx.x < ns) s_cell[threadIdx.x] = a.slotinfo[a.slot0 + s0 + threadIdx.x].y;
+        __syncthreads();
+        for (int t = threadIdx.x; t < ns * R; t += blockDim.x) {
+            const int sl = t / R, r = t - sl * R;
+            const u32x4 x = philox4x32_10(u32x4{(uint32_t)r, (uint32_t)s_cell[sl], (uint32_t)a.snap, 1u}, k0, k1);
+            const float* Ud = lds + sl * noutp;
+            float qd, qb;
+            const int d = mix_search(Ud, sPd, kMlpDir, eps, u01_open(x.x), &qd);
+            const int bg = mix_search(Ud + kMlpDir, sP, nbin, eps, u01_open(x.y), &qb);
+            const int band = bg / a.nQ, g = bg - band * a.nQ;
+            const float ratio = (qb > 0.f && qd > 0.f) ? (a.p_bg[bg] / qb) * ((1.0f / kMlpDir) / qd) : 0.f;
+            const uint32_t mt = ((uint32_t)(d >> 3) << 21) | (x.z >> 11);
+            const uint32_t mp = ((uint32_t)(d & 7) << 21) | (x.w >> 11);
+            uint32_t w0, w1;
+            desc_pack(mt, mp, band, g, &w0, &w1);
+            const long long q = (s0 + sl) * (long long)R + r;
+            a.w0[q] = w0;
+            a.w1[q] = w1;
+            a.w[q] = ratio;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace dn

@@ -43,6 +43,7 @@ __global__ void fill_f64(double* p, long long n, double v);
 // mlp.hip
 __global__ void mlp_proposal(GenArgs a, float* cdf);
 __global__ void raygen_learned(GenArgs a, const float* U, const float* pcdf);
+__global__ void raygen_learned_lds(GenArgs a, const float* U, const float* pcdf, int G);
 __global__ void mlp_train_tile(MlpTrainArgs a);
 __global__ void mlp_reduce(const float* part, const float* loss_part, int nblk, int P, float* grad, float* loss);
 __global__ void mlp_wsum(const float* wt, long long n, float* out);
@@ -654,6 +655,28 @@ static MlpWeights mlp_weights(dnsmc_solver* s)
     return w;
 }
 
+// The learned draws of a batch of nsl slots from their rows U: below 200 rays
+// per slot one thread per ray searching the rows in global memory; from 200 on
+// the rows of G = 800 / R slots (1..8) staged in LDS per block, where the
+// staging is shared by enough rays (DNSMC_RAYGEN_SLOTS = G forces a staging
+// width, 0 the global search).  Both draw the same rays.
+static hipError_t launch_raygen_learned(dnsmc_solver* s, const GenArgs& G, long long nsl, int R, hipStream_t st,
+                                        const float* U, const float* pcdf)
+{
+    const int noutp = G.mlpw.noutp, nbin = G.mlpw.nbin;
+    int gs = R >= 200 ? std::max(1, std::min(8, 800 / R)) : 0;
+    if (const char* e = getenv("DNSMC_RAYGEN_SLOTS")) gs = std::max(0, std::min(8, atoi(e)));
+    const size_t lds_g = sizeof(float) * ((size_t)gs * noutp + nbin);
+    if (gs > 0 && lds_g <= s->lds_max) {
+        const int nb = (int)std::min<long long>((nsl + gs - 1) / gs, (long long)s->ncu * 8);
+        hipLaunchKernelGGL(raygen_learned_lds, dim3(nb), dim3(256), lds_g, st, G, U, pcdf, gs);
+        return hipGetLastError();
+    }
+    const int nb = (int)std::min<long long>((nsl * R + 255) / 256, (long long)s->ncu * 16);
+    hipLaunchKernelGGL(raygen_learned, dim3(nb), dim3(256), sizeof(float) * (size_t)nbin, st, G, U, pcdf);
+    return hipGetLastError();
+}
+
 // Feature arguments of the learned proposal (the field pointer is set per snapshot).
 static FeatArgs feat_args(dnsmc_solver* s, const Derived& d)
 {
@@ -1130,10 +1153,9 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
                 G.winv = nullptr;
                 int nb = (int)std::min<long long>((nsl + 63) / 64, (long long)s->ncu * 3);
                 hipLaunchKernelGGL(mlp_proposal, dim3(nb), dim3(256), lds_prop, gs, G, s->pipe[1].d_qcdf.p);
-                nb = (int)std::min<long long>((nsl * R + 255) / 256, (long long)s->ncu * 16);
-                hipLaunchKernelGGL(raygen_learned, dim3(nb), dim3(256), lds_learn, gs, G,
-                                   (const float*)s->pipe[1].d_qcdf.p, (const float*)s->d_pcdf.p);
                 HIPCHK(hipGetLastError());
+                HIPCHK(launch_raygen_learned(s, G, nsl, R, gs, (const float*)s->pipe[1].d_qcdf.p,
+                                             (const float*)s->d_pcdf.p));
             }
             HIPCHK(hipEventRecord(eg.b, gs));
             HIPCHK(hipEventRecord(s->ev_gen[b], gs));
@@ -1179,9 +1201,8 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
                 const size_t lds_prop = mlp_proposal_lds();
                 int nb = (int)std::min<long long>((nsl + 63) / 64, (long long)s->ncu * 3);
                 hipLaunchKernelGGL(mlp_proposal, dim3(nb), dim3(256), lds_prop, st, G, pp.d_qcdf.p);
-                nb = (int)std::min<long long>((nsl * R + 255) / 256, (long long)s->ncu * 16);
-                hipLaunchKernelGGL(raygen_learned, dim3(nb), dim3(256), lds_learn, st, G, (const float*)pp.d_qcdf.p,
-                                   (const float*)s->d_pcdf.p);
+                HIPCHK(hipGetLastError());
+                HIPCHK(launch_raygen_learned(s, G, nsl, R, st, (const float*)pp.d_qcdf.p, (const float*)s->d_pcdf.p));
             } else if (sorted) {
                 int nb = (int)std::min<long long>(nsl, (long long)s->ncu * 8);
                 hipLaunchKernelGGL(raygen_compat_sorted, dim3(nb), dim3(256), lds_sorted, st, G);
