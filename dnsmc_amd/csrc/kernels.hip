@@ -635,6 +635,10 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     float* s_face = (float*)(s_lev + 6 * kLevWords);
     int* s_off = (int*)(s_face + P.nfaces);     // brick offsets per level and axis
     float* s_ibw = (float*)(s_off + P.noffs);   // 6 * nB wall Planck values
+    // face i of the face table at byte offset fb, brick offset of 1-based
+    // cell c of the table at byte offset ob (both offsets from the level block)
+    auto face_at = [&](int fb, int i) -> float { return *(const float*)((const char*)s_face + fb + (i << 2)); };
+    auto off_at = [&](int ob, int c) -> int { return *(const int*)((const char*)s_off + ob + (c << 2)); };
     float* s_ib = s_ibw + 6 * P.nB;             // Ib rows (nT or tab_nw nodes)
     const int nw = P.tab_nw;
     float* s_kq = s_ib + P.nB * nw;             // sliced kappa rows (kTabSliced)
@@ -645,7 +649,9 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     for (int i = threadIdx.x; i < P.nlev; i += blockDim.x) {
         int* L = s_lev + i * kLevWords;
         L[0] = P.lev[i].n[0]; L[1] = P.lev[i].n[1]; L[2] = P.lev[i].n[2];
-        L[3] = P.lev[i].foff[0]; L[4] = P.lev[i].foff[1]; L[5] = P.lev[i].foff[2];
+        // face and brick-offset table positions as byte offsets into s_face
+        // and s_off (the offsets' -1 of the 1-based cell index folded in)
+        L[3] = 4 * P.lev[i].foff[0]; L[4] = 4 * P.lev[i].foff[1]; L[5] = 4 * P.lev[i].foff[2];
         L[6] = P.lev[i].maxs;
         const float* tb = P.T[i] + (size_t)P.snap * P.lev[i].tstride;
         unsigned long long u = (unsigned long long)tb;
@@ -653,9 +659,9 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         L[8] = (int)(uint32_t)(u >> 32);
         L[9] = P.lev[i].nbx;
         L[10] = P.lev[i].nby;
-        L[11] = P.lev[i].ooff[0];
-        L[12] = P.lev[i].ooff[1];
-        L[13] = P.lev[i].ooff[2];
+        L[11] = 4 * (P.lev[i].ooff[0] - 1);
+        L[12] = 4 * (P.lev[i].ooff[1] - 1);
+        L[13] = 4 * (P.lev[i].ooff[2] - 1);
         L[14] = (int)P.lev[i].tstride;
     }
     for (int i = threadIdx.x; i < P.nfaces; i += blockDim.x) s_face[i] = P.faces[i];
@@ -936,9 +942,9 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                     oo0 = s_lev[11]; oo1 = s_lev[12]; oo2 = s_lev[13];
                 }
                 Tl = T0;
-                fx = s_face[fo0 + cx - (fl & 1)];
-                fy = s_face[fo1 + cy - ((fl >> 1) & 1)];
-                fz = s_face[fo2 + cz - ((fl >> 2) & 1)];
+                fx = face_at(fo0, cx - (fl & 1));
+                fy = face_at(fo1, cy - ((fl >> 1) & 1));
+                fz = face_at(fo2, cz - ((fl >> 2) & 1));
                 idx = rd.idx;
                 tau = 0.0f;
                 tra = 1.0f;
@@ -1186,7 +1192,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
             const int side = cm != 0;
             const int bcbit = ax * 2 + side;
             if ((P.bc_periodic >> bcbit) & 1) {
-                const float pw = s_face[fo + (side ? 0 : nm)];
+                const float pw = face_at(fo, side ? 0 : nm);
                 cm = side ? 1 : nm;
                 px = m0 ? pw : px;
                 py = m1 ? pw : py;
@@ -1196,14 +1202,14 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 cm -= stp;
             }
         }
-        const float fnew = s_face[fo + cm - fla];
+        const float fnew = face_at(fo, cm - fla);
         cx = m0 ? cm : cx;
         cy = m1 ? cm : cy;
         cz = m2 ? cm : cz;
         fx = m0 ? fnew : fx;
         fy = m1 ? fnew : fy;
         fz = m2 ? fnew : fz;
-        idx = s_off[oo0 + cx - 1] + s_off[oo1 + cy - 1] + s_off[oo2 + cz - 1];
+        idx = off_at(oo0, cx) + off_at(oo1, cy) + off_at(oo2, cz);
         const float Tn = load_T(Tl, idx);
         float kap, ibj;
         if (SPEC == kSpecGreyT4) {
@@ -1247,10 +1253,10 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 cx = (cx + 1) >> 1;
                 cy = (cy + 1) >> 1;
                 cz = (cz + 1) >> 1;
-                fx = s_face[fo0 + cx - (fl & 1)];
-                fy = s_face[fo1 + cy - ((fl >> 1) & 1)];
-                fz = s_face[fo2 + cz - ((fl >> 2) & 1)];
-                idx = s_off[oo0 + cx - 1] + s_off[oo1 + cy - 1] + s_off[oo2 + cz - 1];
+                fx = face_at(fo0, cx - (fl & 1));
+                fy = face_at(fo1, cy - ((fl >> 1) & 1));
+                fz = face_at(fo2, cz - ((fl >> 2) & 1));
+                idx = off_at(oo0, cx) + off_at(oo1, cy) + off_at(oo2, cz);
                 Tc = load_T(Tl, idx);
                 cnt = 0;
             } else {
