@@ -582,6 +582,30 @@ __global__ __launch_bounds__(256) void mlp_proposal(GenArgs a, float* U)
     }
 }
 
+// One learned ray: Philox (r, cell, snapshot, 1), the direction bin and the
+// (band, g) bin from the defensive mixtures on the cell's row Ud (mix_search
+// on each head), the direction uniform inside its bin, and p/q into the
+// descriptor words of ray q.
+__device__ __forceinline__ void learned_ray(const GenArgs& a, const float* Ud, const float* sPd, const float* sP,
+                                            uint32_t r, uint32_t cell, long long q)
+{
+    const u32x4 x = philox4x32_10(u32x4{r, cell, (uint32_t)a.snap, 1u}, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+    float qd, qb;
+    const int d = mix_search(Ud, sPd, kMlpDir, a.mix_eps, u01_open(x.x), &qd);
+    const int bg = mix_search(Ud + kMlpDir, sP, a.mlpw.nbin, a.mix_eps, u01_open(x.y), &qb);
+    const int band = bg / a.nQ, g = bg - band * a.nQ;
+    // (a selected bin always has a positive step; the guard keeps a
+    // degenerate row from producing inf * 0)
+    const float ratio = (qb > 0.f && qd > 0.f) ? (a.p_bg[bg] / qb) * ((1.0f / kMlpDir) / qd) : 0.f;
+    const uint32_t mt = ((uint32_t)(d >> 3) << 21) | (x.z >> 11);
+    const uint32_t mp = ((uint32_t)(d & 7) << 21) | (x.w >> 11);
+    uint32_t w0, w1;
+    desc_pack(mt, mp, band, g, &w0, &w1);
+    a.w0[q] = w0;
+    a.w1[q] = w1;
+    a.w[q] = ratio;
+}
+
 // raygen_learned: one thread per ray.  Each ray draws Philox (r, cell,
 // snapshot, 1), its direction bin and (band, g) bin from the defensive
 // mixtures by binary search on its cell's row of U (global memory: the R rays
@@ -599,30 +623,11 @@ __global__ __launch_bounds__(256) void raygen_learned(GenArgs a, const float* U,
     __shared__ float sPd[kMlpDir];
     for (int e = threadIdx.x; e < kMlpDir; e += blockDim.x) sPd[e] = (float)(e + 1) * (1.0f / kMlpDir);
     __syncthreads();
-    const float eps = a.mix_eps;
     const int R = a.rays_per_slot;
-    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     const long long nr = a.nslots * (long long)R;
     for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < nr; q += (long long)gridDim.x * blockDim.x) {
         const long long sl = q / R;
-        const int r = (int)(q - sl * R);
-        const int cell = a.slotinfo[a.slot0 + sl].y;
-        const u32x4 x = philox4x32_10(u32x4{(uint32_t)r, (uint32_t)cell, (uint32_t)a.snap, 1u}, k0, k1);
-        const float* Ud = U + sl * (size_t)noutp;
-        float qd, qb;
-        const int d = mix_search(Ud, sPd, kMlpDir, eps, u01_open(x.x), &qd);
-        const int bg = mix_search(Ud + kMlpDir, sP, nbin, eps, u01_open(x.y), &qb);
-        const int band = bg / a.nQ, g = bg - band * a.nQ;
-        // (a selected bin always has a positive step; the guard keeps a
-        // degenerate row from producing inf * 0)
-        const float ratio = (qb > 0.f && qd > 0.f) ? (a.p_bg[bg] / qb) * ((1.0f / kMlpDir) / qd) : 0.f;
-        const uint32_t mt = ((uint32_t)(d >> 3) << 21) | (x.z >> 11);
-        const uint32_t mp = ((uint32_t)(d & 7) << 21) | (x.w >> 11);
-        uint32_t w0, w1;
-        desc_pack(mt, mp, band, g, &w0, &w1);
-        a.w0[q] = w0;
-        a.w1[q] = w1;
-        a.w[q] = ratio;
+        learned_ray(a, U + sl * (size_t)noutp, sPd, sP, (uint32_t)(q - sl * R), (uint32_t)a.slotinfo[a.slot0 + sl].y, q);
     }
 }
 
@@ -639,10 +644,8 @@ __global__ __launch_bounds__(256) void raygen_learned_lds(GenArgs a, const float
     __shared__ float sPd[kMlpDir];
     for (int e = threadIdx.x; e < kMlpDir; e += blockDim.x) sPd[e] = (float)(e + 1) * (1.0f / kMlpDir);
     __shared__ int s_cell[8];
-    const float eps = a.mix_eps;
     const int R = a.rays_per_slot;
     const long long ngroups = (a.nslots + G - 1) / G;
-    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
         const long long s0 = grp * G;
         const int ns = (int)((a.nslots - s0) < G ? (a.nslots - s0) : G);
@@ -663,21 +666,7 @@ __global__ __launch_bounds__(256) void raygen_learned_lds(GenArgs a, const float
         __syncthreads();
         for (int t = threadIdx.x; t < ns * R; t += blockDim.x) {
             const int sl = t / R, r = t - sl * R;
-            const u32x4 x = philox4x32_10(u32x4{(uint32_t)r, (uint32_t)s_cell[sl], (uint32_t)a.snap, 1u}, k0, k1);
-            const float* Ud = lds + sl * noutp;
-            float qd, qb;
-            const int d = mix_search(Ud, sPd, kMlpDir, eps, u01_open(x.x), &qd);
-            const int bg = mix_search(Ud + kMlpDir, sP, nbin, eps, u01_open(x.y), &qb);
-            const int band = bg / a.nQ, g = bg - band * a.nQ;
-            const float ratio = (qb > 0.f && qd > 0.f) ? (a.p_bg[bg] / qb) * ((1.0f / kMlpDir) / qd) : 0.f;
-            const uint32_t mt = ((uint32_t)(d >> 3) << 21) | (x.z >> 11);
-            const uint32_t mp = ((uint32_t)(d & 7) << 21) | (x.w >> 11);
-            uint32_t w0, w1;
-            desc_pack(mt, mp, band, g, &w0, &w1);
-            const long long q = (s0 + sl) * (long long)R + r;
-            a.w0[q] = w0;
-            a.w1[q] = w1;
-            a.w[q] = ratio;
+            learned_ray(a, lds + sl * noutp, sPd, sP, (uint32_t)r, (uint32_t)s_cell[sl], (s0 + sl) * (long long)R + r);
         }
         __syncthreads();
     }
