@@ -293,7 +293,8 @@ def test_learned_draw_matches_oracle(oracle, trained, monkeypatch, R, slots):
 def test_proposal_rows_match_the_network(trained, monkeypatch):
     """mlp_proposal's cumulative weights per cell against the network forward
     (mlp_forward_rows, pinned by test_forward_matches_oracle) on the same
-    features: each head's row is the running sum of exp(clamp(z, -60, 60))."""
+    features: each head's row is the running sum of exp(clamp(z, -60, 60)),
+    non-decreasing."""
     net, _, tdir, n = trained
     monkeypatch.setenv("DNSMC_SNAPWISE", "0")
     monkeypatch.setenv("DNSMC_LEARNED_FUSED", "0")
@@ -314,6 +315,8 @@ def test_proposal_rows_match_the_network(trained, monkeypatch):
     for lo, hi in ((0, 64), (64, 64 + nbin)):
         want = np.cumsum(e[:, lo:hi], axis=1)
         np.testing.assert_allclose(U[:, lo:hi], want, rtol=2e-4, atol=1e-30)
+        # non-decreasing, so that each bin's draw probability is its step
+        assert np.all(np.diff(U[:, lo:hi].astype(np.float32), axis=1) >= 0)
 
 
 @pytest.mark.parametrize("R", [40, 130])
