@@ -639,19 +639,20 @@ __global__ __launch_bounds__(256) void raygen_learned_lds(GenArgs a, const float
 {
     extern __shared__ float lds[];
     const int noutp = a.mlpw.noutp, nbin = a.mlpw.nbin;
-    float* sP = lds + G * noutp;
+    float* sP = lds + G * noutp + 4;                  // (after the rows and one 16-byte pad slot)
     for (int e = threadIdx.x; e < nbin; e += blockDim.x) sP[e] = pcdf[e];
     __shared__ float sPd[kMlpDir];
     for (int e = threadIdx.x; e < kMlpDir; e += blockDim.x) sPd[e] = (float)(e + 1) * (1.0f / kMlpDir);
     __shared__ int s_cell[8];
     const int R = a.rays_per_slot;
     const long long ngroups = (a.nslots + G - 1) / G;
+    const int npad = G * noutp / 4;                   // the pad slot (float4 index)
     for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
         const long long s0 = grp * G;
         const int ns = (int)((a.nslots - s0) < G ? (a.nslots - s0) : G);
-        // the rows into LDS, eight 16-byte loads in flight per thread
-        // (clamped, always valid indices: no branch between the loads; the
-        // clamped stores repeat the last element's value)
+        // the rows into LDS, eight 16-byte loads in flight per thread (no
+        // branch between them: loads past the rows read the last element
+        // again, their stores go to the pad slot, which nothing reads)
         const float4* src = (const float4*)(U + s0 * (size_t)noutp);
         float4* dst = (float4*)lds;
         const int nv = ns * noutp / 4;
@@ -660,7 +661,10 @@ __global__ __launch_bounds__(256) void raygen_learned_lds(GenArgs a, const float
 #pragma unroll
             for (int u = 0; u < 8; ++u) v[u] = src[min(e0 + u * (int)blockDim.x, nv - 1)];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) dst[min(e0 + u * (int)blockDim.x, nv - 1)] = v[u];
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * (int)blockDim.x;
+                dst[e < nv ? e : npad] = v[u];
+            }
         }
         if ((int)threadIdx.x < ns) s_cell[threadIdx.x] = a.slotinfo[a.slot0 + s0 + threadIdx.x].y;
         __syncthreads();

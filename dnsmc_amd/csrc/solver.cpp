@@ -666,7 +666,7 @@ static hipError_t launch_raygen_learned(dnsmc_solver* s, const GenArgs& G, long 
     const int noutp = G.mlpw.noutp, nbin = G.mlpw.nbin;
     int gs = R >= 200 ? std::max(1, std::min(8, 800 / R)) : 0;
     if (const char* e = getenv("DNSMC_RAYGEN_SLOTS")) gs = std::max(0, std::min(8, atoi(e)));
-    const size_t lds_g = sizeof(float) * ((size_t)gs * noutp + nbin);
+    const size_t lds_g = sizeof(float) * ((size_t)gs * noutp + 4 + nbin);   // rows, a 16-byte pad slot, the plain CDF
     if (gs > 0 && lds_g <= s->lds_max) {
         const int nb = (int)std::min<long long>((nsl + gs - 1) / gs, (long long)s->ncu * 8);
         hipLaunchKernelGGL(raygen_learned_lds, dim3(nb), dim3(256), lds_g, st, G, U, pcdf, gs);
