@@ -169,6 +169,41 @@ def timed_solve(S, steps):
     return (time.perf_counter() - t0) / steps, st_all
 
 
+def shard_leg(S, cfg, steps, full_rays_per_s):
+    """One of 8 shards of the solve, as a multi-GPU run would hand it to one
+    GPU: the emitting cells with i < n0/8 (a slab of the periodic x axis; all
+    j and k), every snapshot, tracing through the full replicated field.  The
+    solver keeps its fields; the emitters are restored to the whole grid."""
+    import dnsmc_amd
+    n0, n1, n2 = cfg.n
+    nx = n0 // 8
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(n1), np.arange(n2), indexing="ij")
+    cells = (i + n0 * (j + n1 * k)).ravel().astype(np.int32)
+    S.set_emitters(cells)
+    S.run()
+    S.wait()
+    dnsmc_amd.sync()
+    nrays = nsteps = 0
+    ms_trace = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        S.run()
+        st = S.stats()
+        nrays += st["n_rays"]
+        nsteps += st["n_steps"]
+        ms_trace += st["ms_trace"]
+    dnsmc_amd.sync()
+    dt = time.perf_counter() - t0
+    S.set_emitters_slab(1)
+    v = nrays / dt
+    return {"emitters": "i in [0, %d) of %d, all j and k: %d cells x %d snapshots" % (nx, n0, len(cells), cfg.nsnap),
+            "rays_per_solve": nrays // steps, "ms_per_solve": 1000.0 * dt / steps,
+            "ms_trace_per_solve": ms_trace / steps, "rays_per_s": v, "segments_per_ray": nsteps / max(nrays, 1),
+            "rays_per_s_over_whole_grid": v / full_rays_per_s,
+            "note": "one GPU's share of an 8-GPU run sharded along x with the field replicated (the field "
+                    "all-gather before and the Q gather after are not part of it)"}
+
+
 def solve_cost(workload, R, fields, mode=0, mlp=None, mix_eps=0.1, **cfg_kw):
     """Full-grid solves of snapshot 0 at R, 4R and 8R rays per cell (each after
     a warm-up solve): the fixed part a and the per-ray-per-cell part b of
@@ -404,6 +439,7 @@ def main():
     ap.add_argument("--copy-gib", type=float, default=8.0, help="GiB of the achievable-bandwidth streams (0: skip)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg and the error study")
     ap.add_argument("--no-configs", action="store_true", help="skip the grey 32^3, 64x96x64 and 256x128x128 keys")
+    ap.add_argument("--no-shard", action="store_true", help="skip the one-of-8 x-shard solve")
     ap.add_argument("--rmax", type=int, default=1 << 23, help="largest rays per cell of the doubling study")
     ap.add_argument("--train-snaps", type=int, default=20)
     ap.add_argument("--train-rays", type=int, default=8)
@@ -518,6 +554,9 @@ def main():
             out["cpu_baseline"] = cpu_baseline(S, cfg, fields, R, args.cpu_seconds)
         except Exception as e:  # reported, never fatal to the GPU number
             log("cpu baseline failed: %r" % (e,))
+    if not args.no_shard and cfg.n[0] % 8 == 0:
+        out["one_shard_of_8"] = shard_leg(S, cfg, args.steps, out["value"])
+        log("shard", out["one_shard_of_8"])
     S.close()
     if not args.no_learned:
         out["accuracy"] = accuracy_legs(args, cfg, R, fields)

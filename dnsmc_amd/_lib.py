@@ -75,6 +75,7 @@ class Stats(C.Structure):
         ("n_rays", C.c_longlong), ("n_steps", C.c_longlong), ("n_trace_launches", C.c_int),
         ("Tmax", C.c_float), ("kappamax", C.c_float), ("Ibmax", C.c_double),
         ("table_mode", C.c_int), ("table_lo", C.c_int), ("table_nw", C.c_int), ("n_pipes", C.c_int),
+        ("n_listed", C.c_longlong),
     ]
 
 

@@ -82,7 +82,7 @@ struct TraceArgs {
     double* E;
     double* A;
     double* Q2;
-    unsigned long long* nsteps;
+    unsigned long long* nsteps;  // [0] segments, [1] rays traced from the long-ray lists
     // several snapshots in one launch (fused generation): rays of snapshot
     // P.snap + s are [0, nrays) again; tally slot = slotinfo.x + s * tally_stride
     int nsnap_launch;
@@ -96,6 +96,18 @@ struct TraceArgs {
     float Ibmax;
     int g_search;
     int gen_lds;                // bit 0: cdf_g staged in LDS too; bit 1: inv_emax too
+    // long rays first (GEN == 1): the rays that the launch would start last
+    // (last snapshot of the launch, final part of each XCD's range: offset >=
+    // cand_from within its cand_fd.d rays) whose polar direction cosine is
+    // near 0 (m_theta within cand_w of 2^23, rays along the walls that may run
+    // to the step cap) are listed by cand_scan; the waves trace the list
+    // first and skip those rays where their super-chunks come up
+    const uint32_t* cand;       // listed ray indices, one list per XCD range (nullptr: off)
+    unsigned* cand_cnt;         // list x: [64 x] rays found (more than cand_cap: off), [64 x + 32] claims
+    uint32_t cand_cap;          // capacity of each list
+    uint32_t cand_w;
+    uint32_t cand_from;
+    FastDiv cand_fd;            // division by the rays of one XCD range (per_x * super)
     int desc_ratio;             // learned descriptors: w holds p/q, the weight is computed at ray start
     // FP64 traversal extras
     const double* inv_emax_d;

@@ -577,6 +577,63 @@ __device__ __noinline__ void learned_row(const uint16_t* W1, const float* b1, co
     }
 }
 
+// Whether ray q of snapshot qs of a fused launch, with polar uniform bits mt
+// (m_theta of the plain draw), is a listed long-ray candidate
+// (TraceArgs::cand): one integer test for cand_scan and the traversal's draw.
+template <typename A>
+__device__ __forceinline__ bool long_candidate(const A& P, uint32_t q, uint32_t qs, uint32_t mt)
+{
+    FastDiv fd;   // (field by field: P may be the kernel argument segment)
+    fd.m = P.cand_fd.m;
+    fd.s = P.cand_fd.s;
+    fd.d = P.cand_fd.d;
+    const uint32_t x = fdiv(q, fd);
+    return qs + 1u == (uint32_t)P.nsnap_launch && q - x * fd.d >= P.cand_from &&
+           mt - (1u << 23) + P.cand_w < 2u * P.cand_w;
+}
+
+// The long-ray candidates of a fused launch: each ray of the final part of
+// the 8 XCD ranges of the launch's last snapshot (nfin rays per range from
+// cand_from on, those below nrays) gets the plain draw's Philox block, and
+// the rays that pass long_candidate go to their range's list (list x at
+// x * cand_cap, its count at cand_cnt[64 x]; every hit is counted, also past
+// cand_cap).  A block takes tiles of kCandTile rays of one range and stages
+// the tile's hits in LDS: one global atomic per tile.
+constexpr uint32_t kCandTile = 4096;
+__global__ __launch_bounds__(256) void cand_scan(TraceArgs P, uint32_t nfin, uint32_t* list)
+{
+    __shared__ uint32_t s_q[kCandTile];
+    __shared__ uint32_t s_n, s_base;
+    const uint32_t qs = (uint32_t)P.nsnap_launch - 1u;
+    const uint32_t tpr = (nfin + kCandTile - 1u) / kCandTile;   // tiles per range
+    for (uint32_t t = blockIdx.x; t < 8u * tpr; t += gridDim.x) {
+        const uint32_t x = t / tpr;
+        const uint32_t j0 = (t - x * tpr) * kCandTile;
+        const uint32_t jn = min(kCandTile, nfin - j0);
+        const uint32_t q0 = x * P.cand_fd.d + P.cand_from + j0;
+        if (threadIdx.x == 0) s_n = 0u;
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < jn; j += blockDim.x) {
+            const uint32_t q = q0 + j;
+            if (q < (uint32_t)P.nrays) {
+                const uint32_t slot = fdiv(q, P.fd_R);
+                const uint32_t r = q - slot * (uint32_t)P.rays_per_slot;
+                const int4 si = P.slotinfo[P.slot0 + slot];
+                const u32x4 h = philox4x32_10(u32x4{r, (uint32_t)si.y, (uint32_t)(P.snap + (int)qs), 0u},
+                                              (uint32_t)P.seed, (uint32_t)(P.seed >> 32));
+                if (long_candidate(P, q, qs, h.z >> 8)) s_q[atomicAdd(&s_n, 1u)] = q;
+            }
+        }
+        __syncthreads();
+        const uint32_t n = s_n;
+        if (threadIdx.x == 0 && n) s_base = atomicAdd(P.cand_cnt + 64u * x, n);
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x)
+            if (s_base + k < P.cand_cap) list[(size_t)x * P.cand_cap + s_base + k] = s_q[k];
+        __syncthreads();   // the tile's hits are out before the next tile's reset
+    }
+}
+
 // Persistent cell-by-cell traversal (code/mc_gpu.cu:228-438).
 //
 // Work: the launch's rays (in 3-D brick order of their emitting cells) are
@@ -618,6 +675,13 @@ __device__ __noinline__ void learned_row(const uint16_t* W1, const float* b1, co
 // its rays, see learned_row), and one launch covers every snapshot: XCD x's
 // claims walk snapshot by snapshot through its 1/8 of each snapshot's
 // super-chunks, so all XCDs work on the same snapshot at a time.
+//
+// Long rays first (GEN == 1, a list from cand_scan): a ray along the walls
+// of an optically thin bin can run to the step cap, tens of ms on its own;
+// started among the launch's last rays it would hold the launch open after
+// all other work is done.  The waves draw their queues from the list first
+// and skip the listed rays (the same test) where their super-chunks come up:
+// every ray is still traced once, with the same arithmetic.
 template <int SPEC, int TAB, int TMODE, bool REC, bool AMC, int GEN>
 __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArgs P)
 {
@@ -774,10 +838,27 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
     float lr_ratio = 0.f;
     uint32_t row_slot = 0xFFFFFFFFu, row_snap = 0xFFFFFFFFu;
     const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    // long rays first, one scalar: bit 8 on (off when a list found more rays
+    // than it holds; with none found no ray passes the test either), bit 31
+    // while the wave still draws from the lists, bits 0..3 the lists it has
+    // used up (its XCD's first, then the following XCDs')
+    uint32_t cand = 0;
+    if (GEN == 1 && QUEUE && P.cand != nullptr) {
+        uint32_t tot = 0;
+        bool fits = true;
+        for (int x = 0; x < 8; ++x) {
+            const uint32_t n = __builtin_amdgcn_readfirstlane(P.cand_cnt[64 * x]);
+            fits = fits && n <= P.cand_cap;
+            tot += n;
+        }
+        cand = fits && tot > 0u ? 0x80000100u : 0u;
+    }
 
     // ray q of snapshot qs of the launch: drawn here (GEN, the plain estimator's
-    // Philox draw, bit-identical to raygen_plain) or read from its descriptors
-    auto draw_ray = [&](uint32_t q, uint32_t qs) -> RayStart {
+    // Philox draw, bit-identical to raygen_plain) or read from its descriptors.
+    // A listed long ray drawn from its super-chunk (not from the list) comes
+    // back with bit 31 of bg set: it is not started there.
+    auto draw_ray = [&](uint32_t q, uint32_t qs, bool listed) -> RayStart {
         RayStart rd;
         const uint32_t slot = fdiv(q, P.fd_R);
         const int4 si = P.slotinfo[P.slot0 + slot];
@@ -795,6 +876,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         uint32_t mt, mp;
         float ratio = 1.0f;       // p/q of a learned descriptor
         bool weigh = GEN;         // emission weight computed here (tables in LDS)
+        bool skip = false;        // a listed long ray, traced from the list
         if (GEN == 1) {
             const uint32_t r = q - slot * (uint32_t)P.rays_per_slot;
             const u32x4 x = philox4x32_10(u32x4{r, (uint32_t)si.y, (uint32_t)(P.snap + (int)qs), 0u},
@@ -809,6 +891,8 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
             g = g > P.nQ - 1 ? P.nQ - 1 : g;
             mt = x.z >> 8;
             mp = x.w >> 8;
+            // (the test's arguments read from the kernel argument segment here)
+            skip = (cand & 0x100u) && !listed && long_candidate(*kernel_args(), q, qs, mt);
         } else if (GEN == 2) {
             bnd = lr_band;
             g = lr_g;
@@ -883,7 +967,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         rd.ix = 1.0f / dx;
         rd.iy = 1.0f / dy;
         rd.iz = 1.0f / dz;
-        rd.bg = bnd | (g << 16) | (f << 24);
+        rd.bg = bnd | (g << 16) | (f << 24) | (skip ? (int)(1u << 31) : 0);
         return rd;
     };
 
@@ -892,7 +976,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
         const unsigned long long m_idle = __ballot(idle);
         const int ni = __popcll(m_idle);
         const int na = 64 - ni;
-        const bool have_rays = more || cur < cend || ghead < gn;
+        const bool have_rays = more || cur < cend || ghead < gn || (int)cand < 0;
         if (na == 0 && !have_rays) break;
         if (have_rays && (ni >= P.refill_k || na == 0)) {
             // ---- tally the rays finished since the last refill
@@ -967,11 +1051,37 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 {
                     const bool s1 = idle && rank < t1;
                     const RayStart rd = pull_ray(Qr, (int)((ghead + (s1 ? rank : 0u)) & 63u));
-                    if (s1) start_ray(rd, gsnap);
+                    if (s1 && (GEN != 1 || rd.bg >= 0)) start_ray(rd, gsnap);
                 }
                 ghead += t1;
                 if (t1 < (uint32_t)ni) {
-                    if (cur >= cend && more) {
+                    bool drawn = false;
+                    if (GEN == 1 && (int)cand < 0) {
+                        // the next (up to) 64 listed long rays, all of the last
+                        // snapshot, from this XCD's list first; no ring entries (their
+                        // slots are scattered): they tally to their slots, or to an
+                        // entry that holds the same slot
+                        const KArgs* K = kernel_args();
+                        while ((int)cand < 0) {
+                            const uint32_t X = (xcd + (cand & 15u)) & 7u;
+                            const uint32_t n = __builtin_amdgcn_readfirstlane(K->cand_cnt[64u * X]);
+                            uint32_t c0 = 0;
+                            if (lane == 0) c0 = atomicAdd(K->cand_cnt + 64u * X + 32u, 64u);
+                            c0 = __builtin_amdgcn_readfirstlane(c0);
+                            if (c0 >= n) {   // used up: the next list, or none left
+                                cand = (cand & 15u) == 7u ? (cand & 0x7FFFFFFFu) : cand + 1u;
+                                continue;
+                            }
+                            gn = min(64u, n - c0);
+                            gsnap = (uint32_t)K->nsnap_launch - 1u;
+                            if (lane == 0) atomicAdd(K->nsteps + 1, (unsigned long long)gn);   // rays listed
+                            if ((uint32_t)lane < gn)
+                                Qr = draw_ray(K->cand[(size_t)X * K->cand_cap + c0 + (uint32_t)lane], gsnap, true);
+                            drawn = true;
+                            break;
+                        }
+                    }
+                    if (!drawn && cur >= cend && more) {
                         // claim the next super-chunk, from this XCD's share first,
                         // then from the following XCDs' shares.  Claim c of XCD X is
                         // chunk X*per_x + c % per_x of snapshot c / per_x.
@@ -996,7 +1106,8 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                         }
                         if (!got) more = false;
                     }
-                    if (cur < cend) {
+                    if (drawn || cur < cend) {
+                      if (!drawn) {
                         gn = min(64u, cend - cur);
                         gsnap = csnap;
                         if (GEN == 2) {
@@ -1061,7 +1172,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                                 K->log_w[o] = lr_ratio;
                             }
                         }
-                        if ((uint32_t)lane < gn) Qr = draw_ray(cur + (uint32_t)lane, gsnap);
+                        if ((uint32_t)lane < gn) Qr = draw_ray(cur + (uint32_t)lane, gsnap, false);
                         cur += gn;
                         // ring entries for the batch's processing slots (consecutive:
                         // lane 0's to lane gn-1's); an entry still holding an older
@@ -1081,9 +1192,10 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                                 s_rv[3 * e] = s_rv[3 * e + 1] = s_rv[3 * e + 2] = 0.0;
                             }
                         }
+                      }
                         const bool s2 = idle && rank >= t1 && rank - t1 < gn;
                         const RayStart rd = pull_ray(Qr, (int)(s2 ? rank - t1 : 0u));
-                        if (s2) start_ray(rd, gsnap);
+                        if (s2 && (GEN != 1 || rd.bg >= 0)) start_ray(rd, gsnap);
                         ghead = min((uint32_t)ni - t1, gn);
                     }
                 }
@@ -1133,7 +1245,7 @@ __global__ __launch_bounds__(TAB == kTabSliced ? 1024 : 256) void trace(TraceArg
                 // ---- start the handed-out rays
                 if (q != 0xFFFFFFFFu) {
                     qcur = q;
-                    start_ray(draw_ray(q, qs), qs);
+                    start_ray(draw_ray(q, qs, false), qs);
                 }
             }
         }

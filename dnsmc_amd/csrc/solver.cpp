@@ -40,6 +40,7 @@ __global__ void spline_pass(SplineArgs s);
 __global__ void reduce_rays(const float2* pray, int tally, long long nslots, int R, long long slot0,
                             const int4* slotinfo, double* E, double* A, double* Q2);
 __global__ void fill_f64(double* p, long long n, double v);
+__global__ void cand_scan(TraceArgs P, uint32_t nfin, uint32_t* list);
 // mlp.hip
 __global__ void mlp_proposal(GenArgs a, float* cdf);
 __global__ void raygen_learned(GenArgs a, const float* U, const float* pcdf);
@@ -101,6 +102,9 @@ constexpr long long kScratchHead = 4096;
 constexpr long long kScratchSlots = (kScratchBytes - kScratchHead) / (3 * (long long)sizeof(double));
 static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
 constexpr int kMaxDevices = 64;
+// long rays first: the window of m_theta (24-bit polar uniform) about 2^23
+// listed by cand_scan, |cos theta| < 0.02
+constexpr uint32_t kCandW = 167772;
 static std::mutex g_scratch_mu;
 static char* g_scratch[kMaxDevices] = {};
 
@@ -222,6 +226,8 @@ struct dnsmc_solver {
         DevBuf<double2> d_pray64;   // per-ray (E, A) of the FP64 traversal
         DevBuf<float> d_qcdf;    // learned proposal: per-slot CDF rows of the batch
         DevBuf<unsigned> d_claim;   // per-XCD super-chunk counters of the traversal
+        DevBuf<uint32_t> d_cand;    // fused launches: the listed long rays
+        DevBuf<unsigned> d_ccnt;    // their count and list claim counter
         hipEvent_t done = nullptr;
     };
     static constexpr int kPipes = 3;
@@ -812,15 +818,26 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
         if (hipMemGetInfo(&fr, &tot) != hipSuccess || need + ((size_t)4 << 30) > fr + have) snapwise = false;
     }
     const long long batch_rays = fused ? 0 : (snapwise ? nslots * R : std::min(slots_per_batch, nslots) * R);
-    const long long nbatch_total = fused ? 1 : (snapwise ? 2 : ((nslots + slots_per_batch - 1) / slots_per_batch) * c.nsnap);
-    int pipes_max = dnsmc_solver::kPipes;   // DNSMC_PIPES=1 serialises the batches (standalone kernel timing)
+    // fused: slots per traversal launch (32-bit ray indices per snapshot)
+    long long fused_part = std::max<long long>(1, ((1ll << 32) - (1ll << 20)) / R);
+    if (const char* fp = getenv("DNSMC_FUSED_SLOTS")) fused_part = std::max<long long>(1, std::min(fused_part, atoll(fp)));
+    const long long nfused = (nslots + fused_part - 1) / fused_part;
+    int pipes_max = dnsmc_solver::kPipes;   // DNSMC_PIPES=1 serialises the launches (standalone kernel timing)
     if (const char* pe = getenv("DNSMC_PIPES")) pipes_max = std::max(1, std::min(dnsmc_solver::kPipes, atoi(pe)));
-    // the snapshot-wise path always uses streams 0 (traversal) and 1 (generation)
-    const int npipes = snapwise ? 2 : (int)std::min<long long>(pipes_max, nbatch_total);
+    // Consecutive traversal launches of one solve alternate over two streams,
+    // each with its own claim counters: a launch's last long rays (up to the
+    // step cap, tens of ms) hold only their own CUs while the next launch fills
+    // the others.  Fused: streams 0 and 1 (the in-kernel learned draw keeps one:
+    // its row cache is indexed by wave).  Snapshot-wise: traversals on streams
+    // 0 and 2, generation on 1.  Batched: up to kPipes pipelined streams.
+    const int npipes = snapwise ? (c.nsnap > 1 && pipes_max >= 3 ? 3 : 2)
+                       : fused  ? (int)std::min<long long>(lfused ? 1 : std::min(pipes_max, 2), nfused)
+                                : (int)std::min<long long>(pipes_max, ((nslots + slots_per_batch - 1) / slots_per_batch) * c.nsnap);
     for (int p = 0; p < npipes && !na; ++p) {
         dnsmc_solver::Pipe& pp = s->pipe[p];
-        // one snapshot needs only descriptor set 0
-        const long long nset = snapwise && p == 1 && c.nsnap < 2 ? 0 : batch_rays;
+        // one snapshot needs only descriptor set 0; stream 2 of the
+        // snapshot-wise path only traces (from the sets of streams 0 and 1)
+        const long long nset = snapwise && ((p == 1 && c.nsnap < 2) || p == 2) ? 0 : batch_rays;
         HIPCHK(pp.d_w0.ensure(nset));
         HIPCHK(pp.d_w1.ensure(nset));
         HIPCHK(pp.d_w.ensure(nset));
@@ -850,7 +867,7 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
         HIPCHK(s->d_Eo.ensure(nemit * c.nsnap));
         HIPCHK(s->d_Ao.ensure(nemit * c.nsnap));
         HIPCHK(s->d_sd.ensure(nemit * c.nsnap));
-        HIPCHK(s->d_nsteps.ensure(1));
+        HIPCHK(s->d_nsteps.ensure(2));
         tE = s->d_E.p;
         tA = s->d_A.p;
         tQ2 = s->d_Q2.p;
@@ -868,7 +885,7 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
     memset(&s->st, 0, sizeof s->st);
     s->t0 = std::chrono::steady_clock::now();
     // counters are cleared by fill kernels (stream-ordered, capture-safe)
-    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s->stream, (unsigned*)nsteps, 2ll, 0u);
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s->stream, (unsigned*)nsteps, 4ll, 0u);
     {
         long long n = nslots * c.nsnap;
         int nb = (int)((n + 255) / 256);
@@ -1115,30 +1132,69 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
                 s->first_batch_rays = nslots * (long long)R;
             }
         }
-        long long part = std::max<long long>(1, ((1ll << 32) - (1ll << 20)) / R);   // slots per launch
-        if (const char* fp = getenv("DNSMC_FUSED_SLOTS")) part = std::max<long long>(1, std::min(part, atoll(fp)));
-        for (long long p0 = 0; p0 < nslots; p0 += part) {
+        // long rays first (the plain in-kernel draw; DNSMC_LONG_FIRST=0: off)
+        bool long_first = !lfused;
+        if (const char* lf = getenv("DNSMC_LONG_FIRST")) long_first = long_first && atoi(lf) != 0;
+        long long li = 0;
+        for (long long p0 = 0; p0 < nslots; p0 += fused_part, ++li) {
+            // launch li on stream li % npipes with that stream's claim counters
+            // (allocation-free run: 256 counters each at words 0 and 512 of the
+            // scratch head)
+            const int pi = (int)(li % npipes);
+            hipStream_t ls = s->pipe[pi].stream;
+            P.claim = na ? claim0 + 512 * pi : s->pipe[pi].d_claim.p;
             P.slot0 = p0;
-            P.nrays = std::min(part, nslots - p0) * R;
-            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, P.claim, 256ll, 0u);
+            P.nrays = std::min(fused_part, nslots - p0) * R;
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, ls, P.claim, 256ll, 0u);
             HIPCHK(hipGetLastError());
+            // long rays first: list this launch's candidates, the rays of the
+            // last snapshot in the final quarter of each of the 8 XCD ranges
+            // that the traversal forms (its grid is a multiple of 8 blocks),
+            // with |cos theta| < 0.02 about the polar axis (a sorting aid only:
+            // any list that matches long_candidate traces every ray once)
+            P.cand = nullptr;
+            P.cand_cnt = nullptr;
+            const uint32_t S = (uint32_t)P.super;
+            const long long per_x = ((P.nrays + S - 1) / S + 7) / 8;
+            if (long_first && !na && 8 * per_x * S < (1ll << 32)) {
+                const long long clo = per_x - (per_x + 3) / 4;
+                const long long nfin = (per_x - clo) * S;
+                P.cand_w = kCandW;
+                P.cand_from = (uint32_t)(clo * S);
+                P.cand_fd = make_fastdiv((uint32_t)(per_x * S));
+                // per-range list capacity: the expected count (2 % of nfin) + 25 % + 512
+                const long long cap = (long long)((double)nfin * (2.0 * kCandW / 16777216.0) * 1.25) + 512;
+                dnsmc_solver::Pipe& cp = s->pipe[pi];
+                HIPCHK(cp.d_cand.ensure(8 * cap));
+                HIPCHK(cp.d_ccnt.ensure(512));
+                hipLaunchKernelGGL(fill_u32, dim3(2), dim3(256), 0, ls, cp.d_ccnt.p, 512ll, 0u);
+                P.cand = cp.d_cand.p;
+                P.cand_cnt = cp.d_ccnt.p;
+                P.cand_cap = (uint32_t)cap;
+                const int nb = (int)std::min<long long>(8 * ((nfin + 4095) / 4096), (long long)s->ncu * 8);
+                hipLaunchKernelGGL(cand_scan, dim3(nb), dim3(256), 0, ls, P, (uint32_t)nfin, cp.d_cand.p);
+                HIPCHK(hipGetLastError());
+            }
             const long long nsc = ((P.nrays + P.super - 1) / P.super) * c.nsnap;
             const int tb_cap = (int)std::min<long long>(trace_blocks, std::max<long long>(8, (nsc + 15) / 16));
             EventPair* et = na ? nullptr : &next_event(s, 1);
-            if (et) HIPCHK(hipEventRecord(et->a, st));
-            HIPCHK(launch_trace(spec, tab, lfused ? 2 : 1, false, false, amc, s->ncu, tb_cap, lds_trace, st, P));
-            if (et) HIPCHK(hipEventRecord(et->b, st));
+            if (et) HIPCHK(hipEventRecord(et->a, ls));
+            HIPCHK(launch_trace(spec, tab, lfused ? 2 : 1, false, false, amc, s->ncu, tb_cap, lds_trace, ls, P));
+            if (et) HIPCHK(hipEventRecord(et->b, ls));
             s->st.n_trace_launches++;
         }
     }
     if (snapwise) {
         // generation on pipe 1 into descriptor set (snap % 2), one traversal
         // launch per snapshot on the main stream
-        hipStream_t gs = s->pipe[1].stream, ts = s->pipe[0].stream;
+        hipStream_t gs = s->pipe[1].stream;
         const size_t lds_prop = mlp_proposal_lds();
         for (int snap = 0; snap < c.nsnap; ++snap) {
             const int b = snap & 1;
             dnsmc_solver::Pipe& buf = s->pipe[b];
+            // odd snapshots trace on stream 2 when there is one
+            dnsmc_solver::Pipe& tp = s->pipe[b && npipes > 2 ? 2 : 0];
+            hipStream_t ts = tp.stream;
             if (snap >= 2) HIPCHK(hipStreamWaitEvent(gs, s->ev_tr[b], 0));   // trace(snap - 2) done with the set
             EventPair& eg = next_event(s, 0);
             HIPCHK(hipEventRecord(eg.a, gs));
@@ -1169,8 +1225,8 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
             P.A = s->d_A.p + (size_t)snap * nslots;
             P.Q2 = s->d_Q2.p + (size_t)snap * nslots;
             P.nrec = 0;
-            P.claim = s->pipe[0].d_claim.p;
-            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, ts, s->pipe[0].d_claim.p, 256ll, 0u);
+            P.claim = tp.d_claim.p;
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, ts, tp.d_claim.p, 256ll, 0u);
             HIPCHK(hipGetLastError());
             const long long nsc = (P.nrays + P.super - 1) / P.super;
             const int tb_cap = (int)std::min<long long>(trace_blocks, std::max<long long>(8, (nsc + 15) / 16));
@@ -1304,9 +1360,10 @@ static int do_wait(dnsmc_solver* s)
         else if (s->evs[i].kind == 1) s->st.ms_trace += ms;
         else s->st.ms_other += ms;
     }
-    unsigned long long ns = 0;
-    HIPCHK(hipMemcpy(&ns, s->nsteps_src, sizeof ns, hipMemcpyDeviceToHost));
-    s->st.n_steps = (long long)ns;
+    unsigned long long ns[2] = {0, 0};
+    HIPCHK(hipMemcpy(ns, s->nsteps_src, sizeof ns, hipMemcpyDeviceToHost));
+    s->st.n_steps = (long long)ns[0];
+    s->st.n_listed = (long long)ns[1];
     return DNSMC_OK;
 }
 

@@ -138,6 +138,7 @@ typedef struct dnsmc_stats {
     int table_mode;           /* per-step table reads: 0 global, 1 Ib in LDS, 2 kappa+Ib slice in LDS */
     int table_lo, table_nw;   /* temperature nodes held in LDS (mode 2) */
     int n_pipes;              /* batches in flight at once (one stream each); their traversal launches overlap */
+    long long n_listed;       /* rays traced first from the long-ray lists (fused plain solves) */
 } dnsmc_stats;
 
 /* Defaults equal to the reference's shipped code/param.h + NarrowBand.h. */
@@ -218,7 +219,9 @@ int dnsmc_set_emitters_list(dnsmc_solver *s, const int *cells, int ncells);
 /* Queue the whole radiation solve (all snapshots) and return.
  * The plain estimator with FP64 atomic tallies and records off draws its
  * rays inside the traversal, one launch for all snapshots (one per range of
- * consecutive slots beyond 2^32 - 2^20 rays per snapshot); the learned
+ * consecutive slots beyond 2^32 - 2^20 rays per snapshot, alternating over
+ * two streams), the rays along the walls that it would start last traced
+ * first (dnsmc_stats.n_listed); the learned
  * estimator with FP64 atomic tallies and records off generates the
  * descriptors of a whole snapshot on a second stream while the previous
  * snapshot is traced (one traversal launch per snapshot, two snapshot-sized
@@ -228,8 +231,11 @@ int dnsmc_set_emitters_list(dnsmc_solver *s, const int *cells, int ncells);
  * defaults are the measured best):
  *   DNSMC_FUSED=0     batched generation for the plain estimator too
  *   DNSMC_FUSED_SLOTS=n  at most n slots per fused launch
+ *   DNSMC_LONG_FIRST=0   fused launches trace in claim order only (no
+ *                     list of the rays along the walls traced first)
  *   DNSMC_SNAPWISE=0  batched generation for the learned estimator too
- *   DNSMC_PIPES=1..3  pipeline streams of the batched modes
+ *   DNSMC_PIPES=1..3  pipeline streams of the batched modes (1: the
+ *                     launches of a solve also stay on one stream)
  *   DNSMC_TABLES=global  keep kappa / Planck tables out of LDS
  *   DNSMC_REFILL_K=k  idle lanes of a wave before it starts new rays
  *   DNSMC_SUPER=n     rays per claimed super-chunk (multiple of 64)

@@ -68,6 +68,27 @@ def test_noalloc_matches_run_twice(ck_table_dir):
     S.close()
 
 
+def test_noalloc_split_launches(ck_table_dir, monkeypatch):
+    """A solve cut into many traversal launches (97 slots each, the split a
+    snapshot of more than 2^32 rays takes): consecutive launches alternate over
+    two streams with the claim counters at two places of the scratch array;
+    the results equal dnsmc_run's, split the same way and whole."""
+    S = _solver(ck_table_dir[0])
+    ncell = N[0] * N[1] * N[2]
+    S.run()
+    whole = [S.results(sn) for sn in range(2)]
+    steps_whole = S.stats()["n_steps"]
+    monkeypatch.setenv("DNSMC_FUSED_SLOTS", "97")
+    S.run()
+    ref = [S.results(sn) for sn in range(2)]
+    st = S.stats()
+    assert st["n_trace_launches"] == -(-ncell // 97)
+    _same([r["Q"] for r in ref], [r["sdev"] for r in ref], whole, st["n_steps"], steps_whole)
+    Q, V = _noalloc(S, ncell, 2)
+    _same(Q, V, ref, S.stats()["n_steps"], st["n_steps"])
+    S.close()
+
+
 def test_noalloc_refusals(ck_table_dir):
     # nothing sized yet: refused with the buffer named, the solver still usable
     S = _solver(ck_table_dir[0], nsnap=1)
