@@ -1135,6 +1135,12 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
         // long rays first (the plain in-kernel draw; DNSMC_LONG_FIRST=0: off)
         bool long_first = !lfused;
         if (const char* lf = getenv("DNSMC_LONG_FIRST")) long_first = long_first && atoi(lf) != 0;
+        // the listed part, the final 1/long_part of each XCD range of the last
+        // snapshot: a half in launches of several snapshots, a quarter in
+        // one-snapshot launches (measured on 384x256x256: 8 snapshots 3200 /
+        // 3211 / 3205 ms for a half / quarter / all, one snapshot 411 / 409 / 417)
+        long long long_part = c.nsnap > 1 ? 2 : 4;
+        if (const char* lp = getenv("DNSMC_LONG_PART")) long_part = std::max(1, atoi(lp));
         long long li = 0;
         for (long long p0 = 0; p0 < nslots; p0 += fused_part, ++li) {
             // launch li on stream li % npipes with that stream's claim counters
@@ -1148,7 +1154,7 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
             hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, ls, P.claim, 256ll, 0u);
             HIPCHK(hipGetLastError());
             // long rays first: list this launch's candidates, the rays of the
-            // last snapshot in the final quarter of each of the 8 XCD ranges
+            // last snapshot in the final part of each of the 8 XCD ranges
             // that the traversal forms (its grid is a multiple of 8 blocks),
             // with |cos theta| < 0.02 about the polar axis (a sorting aid only:
             // any list that matches long_candidate traces every ray once)
@@ -1157,7 +1163,7 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
             const uint32_t S = (uint32_t)P.super;
             const long long per_x = ((P.nrays + S - 1) / S + 7) / 8;
             if (long_first && !na && 8 * per_x * S < (1ll << 32)) {
-                const long long clo = per_x - (per_x + 3) / 4;
+                const long long clo = per_x - (per_x + long_part - 1) / long_part;
                 const long long nfin = (per_x - clo) * S;
                 P.cand_w = kCandW;
                 P.cand_from = (uint32_t)(clo * S);

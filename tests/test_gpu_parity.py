@@ -505,12 +505,13 @@ def test_fused_generation_matches_batched(ck_table_dir, monkeypatch, spectral, g
 @pytest.mark.parametrize("grid_num", [1, 2])
 def test_long_rays_first_changes_only_the_order(ck_table_dir, monkeypatch, grid_num):
     """Long rays first: a fused launch lists the rays it would start last (the
-    final quarter of each of the 8 XCD ranges of its last snapshot) whose
+    final half of each of the 8 XCD ranges of its last snapshot, with several
+    snapshots per launch) whose
     polar direction cosine is below 0.02 in magnitude, traces those first and
     skips them where their super-chunks come up.  Against DNSMC_LONG_FIRST=0:
     every ray traced once (equal step totals), tallies equal up to the order
     of the FP64 atomic additions, and the listed rays about 2 % of that final
-    quarter (binomial count, 5 sigma)."""
+    half (binomial count, 5 sigma)."""
     tdir, _ = ck_table_dir
     n, R, ns = (16, 24, 16), 48, 2
     maxs = (6, 100000, 100000, 100000, 100000, 100000)
@@ -529,10 +530,10 @@ def test_long_rays_first_changes_only_the_order(ck_table_dir, monkeypatch, grid_
         S.close()
     (s0, r0), (s1, r1) = out["0"], out["1"]
     assert s0["n_listed"] == 0
-    # the final quarter of the 8 XCD ranges of 512-ray super-chunks
+    # the final half of the 8 XCD ranges of 512-ray super-chunks
     nrays, sc = n[0] * n[1] * n[2] * R, 512
     per_x = -(-(-(-nrays // sc)) // 8)
-    clo, D = per_x - (-(-per_x // 4)), per_x * sc
+    clo, D = per_x - (-(-per_x // 2)), per_x * sc
     fin = sum(max(0, min((x + 1) * D, nrays) - (x * D + clo * sc)) for x in range(8))
     p = 2 * 167772 / 2.0 ** 24
     assert abs(s1["n_listed"] - fin * p) < 5 * math.sqrt(fin * p * (1 - p)), (s1["n_listed"], fin * p)
