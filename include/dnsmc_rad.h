@@ -251,10 +251,13 @@ int dnsmc_run(dnsmc_solver *s);
 /* Allocation-free variant of dnsmc_run.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  It queues
- * the plain estimator's solve of every snapshot exactly as dnsmc_run does
- * (rays drawn inside the traversal; mode 0, precision 0, tally 0, records
- * off, else DNSMC_EINVAL) and calls no hipMalloc, hipFree or other allocating
- * routine: the tallies, work-claim counters and step counter live in a static
+ * the plain estimator's solve of every snapshot as dnsmc_run does (rays
+ * drawn inside the traversal; mode 0, precision 0, tally 0, records off,
+ * else DNSMC_EINVAL), in plain claim order (the long-ray list needs a buffer:
+ * the same rays, the FP64 sums in another order), and calls no hipMalloc,
+ * hipFree or other allocating routine: the tallies, work-claim counters
+ * (two sets, for launches alternating over two streams) and the step
+ * counter live in a static
  * 64 MiB device array of the library (one per GPU, held until the process
  * ends), which limits a call to 2,796,032 emitters x snapshots (DNSMC_EINVAL
  * beyond).  Results go to the caller's device arrays Q[nsnap][nemit] and
