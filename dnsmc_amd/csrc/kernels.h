@@ -24,10 +24,16 @@ struct LevParams {
 
 // 4x4x4-bricked field index of 0-based cell (x, y, z): bricks in (z, y, x)
 // order, 64 contiguous floats per brick.  A ray moving along any axis stays
-// inside the same 256-byte brick for 3 of 4 steps.
+// inside the same 256-byte brick for 3 of 4 steps.  Inside a brick: x bit 0,
+// y, z, then x bit 1, so each 128-byte line holds 2 x 4 x 4 cells (x, y, z):
+// of the three ways to halve a brick this one crosses the fewest line
+// boundaries per unit path on the channel grids, whose cells are thinnest in
+// y, then z, then x (crossings ~ sum over axes of 1 / (cells per line x cell
+// size)); measured 0.75 % faster than 4 x 4 x 2 lines.
 __host__ __device__ __forceinline__ int brick_index(int x, int y, int z, int nbx, int nby)
 {
-    return ((((z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)) << 6) | ((z & 3) << 4) | ((y & 3) << 2) | (x & 3);
+    return ((((z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)) << 6) | (((x >> 1) & 1) << 5) | ((z & 3) << 3) |
+           ((y & 3) << 1) | (x & 1);
 }
 
 enum { kSpecTable = 0, kSpecGreyT4 = 1 };
