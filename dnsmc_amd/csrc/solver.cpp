@@ -1169,7 +1169,9 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
                 P.cand_from = (uint32_t)(clo * S);
                 P.cand_fd = make_fastdiv((uint32_t)(per_x * S));
                 // per-range list capacity: the expected count (2 % of nfin) + 25 % + 512
-                const long long cap = (long long)((double)nfin * (2.0 * kCandW / 16777216.0) * 1.25) + 512;
+                // (DNSMC_LONG_CAP=n: at most n, to exercise the overflow path)
+                long long cap = (long long)((double)nfin * (2.0 * kCandW / 16777216.0) * 1.25) + 512;
+                if (const char* lc = getenv("DNSMC_LONG_CAP")) cap = std::max(1ll, std::min(cap, atoll(lc)));
                 dnsmc_solver::Pipe& cp = s->pipe[pi];
                 HIPCHK(cp.d_cand.ensure(8 * cap));
                 HIPCHK(cp.d_ccnt.ensure(512));

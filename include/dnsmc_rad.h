@@ -236,6 +236,8 @@ int dnsmc_set_emitters_list(dnsmc_solver *s, const int *cells, int ncells);
  *   DNSMC_LONG_PART=k    list those rays from the final 1/k of each XCD's
  *                     share of the last snapshot (default 2 with several
  *                     snapshots per launch, 4 with one)
+ *   DNSMC_LONG_CAP=n     at most n rays per list (a list that finds more
+ *                     turns the listing off for the launch: plain order)
  *   DNSMC_SNAPWISE=0  batched generation for the learned estimator too
  *   DNSMC_PIPES=1..3  pipeline streams of the batched modes (1: the
  *                     launches of a solve also stay on one stream)

@@ -543,6 +543,36 @@ def test_long_rays_first_changes_only_the_order(ck_table_dir, monkeypatch, grid_
         np.testing.assert_allclose(r1[s]["A"], r0[s]["A"], rtol=1e-12, atol=0)
 
 
+def test_long_ray_list_overflow_falls_back_to_plain_order(ck_table_dir, monkeypatch):
+    """Lists of 8 rays (DNSMC_LONG_CAP=8) against the ~370 that each XCD
+    range's half finds: the launch turns the listing off (nothing listed,
+    nothing skipped) and traces every ray in plain claim order, so the step
+    totals and tallies equal those of DNSMC_LONG_FIRST=0."""
+    tdir, _ = ck_table_dir
+    n, R, ns = (16, 24, 16), 48, 2
+    out = {}
+    for key, env in (("off", {"DNSMC_LONG_FIRST": "0"}), ("overflow", {"DNSMC_LONG_CAP": "8"})):
+        for k in ("DNSMC_LONG_FIRST", "DNSMC_LONG_CAP"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        cfg = workloads.channel_config(n, rays_per_cell=R, spectral=0, Tw=573.0, table_dir=tdir, nsnap=ns)
+        S = dnsmc_amd.Solver(cfg)
+        S.load_tables()
+        for s in range(ns):
+            S.set_field(s, workloads.ghosted(workloads.field_fourier(90 + s, n)))
+        S.run()
+        st = S.stats()
+        out[key] = (st, [S.results(s) for s in range(ns)])
+        S.close()
+    (s0, r0), (s1, r1) = out["off"], out["overflow"]
+    assert s0["n_listed"] == 0 and s1["n_listed"] == 0
+    assert s1["n_steps"] == s0["n_steps"]
+    for s in range(ns):
+        np.testing.assert_allclose(r1[s]["E"], r0[s]["E"], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(r1[s]["A"], r0[s]["A"], rtol=1e-12, atol=0)
+
+
 def test_fp64_traversal_with_coarsening_matches_fp64_oracle(oracle, ck_table_dir):
     """precision 1 through two coarsening levels (5 fine steps, then the
     coarse spline field): tallies against the oracle's FP64 estimator."""
