@@ -671,16 +671,19 @@ static MlpWeights mlp_weights(dnsmc_solver* s)
     return w;
 }
 
-// The learned draws of a batch of nsl slots from their rows U: below 200 rays
-// per slot one thread per ray searching the rows in global memory; from 200 on
-// the rows of G = 800 / R slots (1..8) staged in LDS per block, where the
-// staging is shared by enough rays (DNSMC_RAYGEN_SLOTS = G forces a staging
-// width, 0 the global search).  Both draw the same rays.
+// The learned draws of a batch of nsl slots from their rows U: the rows of G
+// slots staged in LDS per block (G = 4 below 200 rays per slot,
+// 3 from 200 on), or one thread per ray searching the rows in global memory
+// (DNSMC_RAYGEN_SLOTS = G forces a staging width, 0 the global search).  Both
+// draw the same rays.  Measured, one coarsened 384x256x256 learned solve:
+// 50 rays/cell 257 ms global, 257 / 235 / 249 / 255 ms for G = 2 / 4 / 6 /
+// 8; 100: 411 global, 388 / 375 / 374 / 395 for G = 3 / 4 / 5 / 6; 200:
+// 639 / 636 / 647 for G = 2 / 3 / 4; 400: 1198 / 1173 / 1163 for G = 1 / 2 / 3.
 static hipError_t launch_raygen_learned(dnsmc_solver* s, const GenArgs& G, long long nsl, int R, hipStream_t st,
                                         const float* U, const float* pcdf)
 {
     const int noutp = G.mlpw.noutp, nbin = G.mlpw.nbin;
-    int gs = R >= 200 ? std::max(1, std::min(8, 800 / R)) : 0;
+    int gs = R < 200 ? 4 : 3;
     if (const char* e = getenv("DNSMC_RAYGEN_SLOTS")) gs = std::max(0, std::min(8, atoi(e)));
     const size_t lds_g = sizeof(float) * ((size_t)gs * noutp + 4 + nbin);   // rows, a 16-byte pad slot, the plain CDF
     if (gs > 0 && lds_g <= s->lds_max) {

@@ -255,15 +255,15 @@ def test_snapshotwise_learned_matches_batched(trained, monkeypatch, nsnap, pipes
         np.testing.assert_allclose(r1[s]["A"], r0[s]["A"], rtol=1e-12, atol=0)
 
 
-@pytest.mark.parametrize("R,slots", [(40, None), (40, "3"), (256, None)])
+@pytest.mark.parametrize("R,slots", [(40, None), (40, "0"), (40, "3"), (256, None)])
 def test_learned_draw_matches_oracle(oracle, trained, monkeypatch, R, slots):
     """The learned generator ray by ray against the oracle's learned draw
     (orc_learned_draw, pinned on the CPU by the plain limit and the mixture
     frequencies) on the same cumulative-weight rows: descriptor words bit for
-    bit and p/q to the last bit (the same FP32 operations).  40 rays per cell
-    search the rows in global memory (raygen_learned), 256 stage two cells'
-    rows per block in LDS (raygen_learned_lds), and DNSMC_RAYGEN_SLOTS=3
-    forces three per block at 40 (a partial last group)."""
+    bit and p/q to the last bit (the same FP32 operations).  By default 40
+    rays per cell stage four cells' rows per block in LDS and 256 three
+    (raygen_learned_lds); DNSMC_RAYGEN_SLOTS=0 forces the search of the rows
+    in global memory (raygen_learned), =3 three cells per block at 40."""
     net, _, tdir, n = trained
     monkeypatch.setenv("DNSMC_SNAPWISE", "0")
     monkeypatch.setenv("DNSMC_LEARNED_FUSED", "0")
