@@ -1023,10 +1023,15 @@ static int do_run(dnsmc_solver* s, const NoAlloc* na)
     P.fd_R = make_fastdiv((uint32_t)R);
     {
         // measured defaults: launches with the ray queue (atomic tallies, no
-        // records) refill at 24 idle lanes, the others at 16; the fused and
+        // records) refill at 24 idle lanes (the learned estimator's long rays
+        // on the fine grid alone at 40: one 384x256x256 snapshot at 100
+        // rays/cell traced in 1696 / 1671 / 1653 / 1710 ms at 24 / 32 / 40 /
+        // 48, against 401 / 403 / 438 ms plain at 24 / 32 / 40 and 263 / 271
+        // ms learned coarsened at 24 / 32), the others at 16; the fused and
         // snapshot-wise launches hand out 512-ray super-chunks, the batched 256
         const char* rk = getenv("DNSMC_REFILL_K");
-        P.refill_k = rk ? atoi(rk) : (!det && !s->record ? 24 : 16);
+        const int rk_queue = c.mode == DNSMC_MODE_LEARNED && P.nlev == 1 ? 40 : 24;
+        P.refill_k = rk ? atoi(rk) : (!det && !s->record ? rk_queue : 16);
         if (P.refill_k < 1) P.refill_k = 1;
         if (P.refill_k > 64) P.refill_k = 64;
         const char* sc = getenv("DNSMC_SUPER");
