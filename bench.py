@@ -151,8 +151,15 @@ def measured_traffic(args, cfg, R):
     want = "%s rays=%d nsnap=%d batch=%s" % (args.workload, R, cfg.nsnap, args.batch)
     if kv.get("workload") != want or "traffic_bytes_per_launch" not in kv:
         return None
-    return {"traffic_bytes_per_launch": float(kv["traffic_bytes_per_launch"]), "l2_hit_rate": float(kv["l2_hit_rate"]),
-            "source": os.path.relpath(path, ROOT)}
+    out = {"traffic_bytes_per_launch": float(kv["traffic_bytes_per_launch"]), "l2_hit_rate": float(kv["l2_hit_rate"]),
+           "source": os.path.relpath(path, ROOT)}
+    # the co-limits the same counter runs show: VALU instructions issued per
+    # CU-cycle (at most 1: four 16-lane SIMDs), texture-data unit busy, L2
+    # requests and misses per segment
+    for k in ("valu_issue_per_cu_cycle", "td_busy_per_cu_cycle", "l2_requests_per_segment", "l2_misses_per_segment"):
+        if k in kv:
+            out[k] = float(kv[k])
+    return out
 
 
 def timed_solve(S, steps):
@@ -524,6 +531,9 @@ def main():
         out["roofline"]["traffic"] = tb
         out["roofline"]["traffic_source"] = prof["source"]
         out["roofline"]["l2_hit_rate"] = prof["l2_hit_rate"]
+        for k in ("valu_issue_per_cu_cycle", "td_busy_per_cu_cycle", "l2_requests_per_segment", "l2_misses_per_segment"):
+            if k in prof:
+                out["roofline"][k] = prof[k]
         # measured HBM traffic (cache-line granular gathers, L2 misses) per second
         out["roofline"]["traffic_GBs_per_launch"] = tb / (ms_trace / max(launches, 1) * 1e-3) / 1e9
         out["roofline"]["traffic_GBs_whole_device"] = tb * launches / dt / 1e9
